@@ -1,0 +1,148 @@
+/*
+ * neurograph_hip.h -- C ABI of libng_hip.so, the MI355X (gfx950) path for neurograph's all-pairs
+ * shortest paths and Brandes betweenness centrality.
+ *
+ * Plain C: pointers, sizes and enums only. Every call is blocking: it enqueues on the context's HIP
+ * stream and synchronizes before returning. Host buffers are borrowed for the duration of a call;
+ * device copies are owned by the returned handles. Errors are status codes; ng_last_error() returns
+ * a thread-local message for the last failing call on the calling thread. One host thread per
+ * context; calls on one graph handle are not re-entrant.
+ *
+ * Citations are file:line in the reference checkout (tuda-hpclab/neurograph).
+ */
+#ifndef NEUROGRAPH_HIP_H
+#define NEUROGRAPH_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    NG_OK = 0,
+    NG_EINVAL = 1,       /* bad argument: node id out of range, malformed CSR, negative raw weight, ... */
+    NG_ENOMEM = 2,       /* device or host allocation failed */
+    NG_EDEVICE = 3,      /* HIP runtime error or a device-side consistency check failed */
+    NG_EUNSUPPORTED = 4  /* valid request this path does not implement (e.g. zero-length arcs in Brandes) */
+} ng_status;
+
+/* Arc weight storage. NG_W_I32 is the reference's weight_type (source/Types.h:31);
+ * NG_W_F32 / NG_W_F64 are real-valued weights, an extension of the reference's integer-only loader. */
+typedef enum { NG_W_I32 = 0, NG_W_F32 = 1, NG_W_F64 = 2 } ng_weight_kind;
+
+/* Arc length used by the shortest-path metric.
+ *   NG_LEN_ABS      |w|, DijkstraCommon::transform_weight<distance_type> (DijkstraCommon.h:43-46)
+ *   NG_LEN_INVERSE  1/|w| in double, 0 -> infinitely long (DijkstraCommon.h:48-52)
+ *   NG_LEN_RAW      w itself, which must be >= 0: delta-stepping's safe_cast<distance_type>(weight)
+ *                   (PushCheckVariant.h:82) throws on negative weights, here NG_EINVAL. */
+typedef enum { NG_LEN_ABS = 0, NG_LEN_INVERSE = 1, NG_LEN_RAW = 2 } ng_length_kind;
+
+/* Distance element type of dist_out buffers (ng_graph_dist_kind). Unreachable = the type's max(),
+ * as in the reference (Dijkstra.h:63, 71). */
+typedef enum { NG_DIST_U64 = 0, NG_DIST_F32 = 1, NG_DIST_F64 = 2 } ng_dist_kind;
+
+/* Per-source result: SsspResult<d_type> (metrics/paths/types/SsspResult.h:19-25) with exact integer
+ * channels. sum_eff is the sum of 1/d over reached nodes with d != 0 (SsspResult.h:56-58).
+ * Real-valued distance sums are accumulated in double. */
+typedef struct {
+    double sum_d;
+    uint64_t sum_d_u64;   /* integer lengths only, else 0 */
+    double max_d;
+    uint64_t max_d_u64;   /* integer lengths only, else 0 */
+    double sum_eff;
+    uint64_t n_unreached;
+} ng_sssp_summary;
+
+/* Timing of the most recent call on a graph (HIP events on the context stream, around the batched
+ * path kernel only) plus work counters of that call. */
+typedef struct {
+    double kernel_ms;        /* sum of the hot kernel's launch durations in the last call */
+    uint32_t launches;       /* number of hot-kernel launches in the last call */
+    uint32_t slots;          /* concurrent source slots (workgroups) used */
+    double delta;            /* bucket width used */
+    uint64_t arcs_reached;   /* sum over sources of out-arcs of reached nodes (TEPS numerator) */
+} ng_call_stats;
+
+typedef struct ng_context ng_context;
+typedef struct ng_graph ng_graph;
+typedef struct ng_cohort ng_cohort;
+
+/* Opens HIP device `device` and creates one stream (default priority). Replaces the per-rank MPI
+ * set-up: MPIWrapper::init (external/mpi-wrapper/MPIWrapper.h:88-109). */
+ng_status ng_context_create(int device, ng_context** out);
+void ng_context_destroy(ng_context* ctx);
+
+/* Uploads a graph given as host CSR over out-arcs: out_offsets[n+1] (non-decreasing, out_offsets[0]=0,
+ * out_offsets[n]=m), out_targets[m] < n, out_weights[m] of `wkind`. Replaces the per-rank RMA windows
+ * and remote adjacency cache: DistributedGraph::construct_graph (DistributedGraph.cpp:46-127),
+ * get_out_arcs (DistributedGraph.h:275-311). Parallel arcs and self-loops are kept (they are arcs of
+ * the reference graph too); m must be < 2^32 (arc_id_type, Types.h:27). */
+ng_status ng_graph_create(ng_context* ctx, uint32_t n, uint64_t m, const uint64_t* out_offsets,
+                          const uint32_t* out_targets, const void* out_weights, ng_weight_kind wkind,
+                          ng_graph** out);
+void ng_graph_destroy(ng_graph* g);
+
+/* Distance element type of dist_out for this graph and length kind. */
+ng_status ng_graph_dist_kind(const ng_graph* g, ng_length_kind lkind, ng_dist_kind* out);
+
+/* Multi-source SSSP with per-source summaries, one batched launch for all sources. Replaces the
+ * per-source loop of AllPairsShortestPath::compute_apsp_impl (AllPairsShortestPath.h:130-143) over
+ * Dijkstra::compute_sssp (Dijkstra.h:56-98) + summarize_distances (SsspResult.h:43-64), and the
+ * delta-stepping dispatcher DeltaStepping::compute_shortest_distances (DeltaStepping.h:36-82).
+ * delta <= 0 selects the graph's tuned (or heuristic) bucket width. out[n_sources] is required;
+ * dist_out (n_sources * n elements of ng_graph_dist_kind) is optional. */
+ng_status ng_sssp_summaries(ng_graph* g, const uint32_t* sources, uint32_t n_sources, ng_length_kind lkind,
+                            double delta, ng_sssp_summary* out, void* dist_out);
+
+/* Brandes betweenness over `sources` with lengths |w|, accumulated into bc_inout[n] like
+ * local_betweenness in BrandesVariant::accumulate_betweenness (BrandesVariant.h:50-74, :157) and the
+ * dispatcher BetweennessCentrality::compute_average_betweenness_centrality (BetweennessCentrality.h:46-80).
+ * out_opt[n_sources] (optional) receives the per-source SSSP summaries of the same forward pass, so
+ * one call serves avg-apsp and avg-centrality. Zero-length arcs are NG_EUNSUPPORTED (BrandesVariant.h:30-31). */
+ng_status ng_brandes_accumulate(ng_graph* g, const uint32_t* sources, uint32_t n_sources, double delta,
+                                double* bc_inout, ng_sssp_summary* out_opt);
+
+/* Edge-level path counts: edge_inout[m] += sum over sources of sigma(v) * (1 + delta(w)) / sigma(w) for every
+ * shortest-path arc (v,w), the per-arc term of BrandesVariant.h:151-154, indexed like out_targets. */
+ng_status ng_edge_dependency(ng_graph* g, const uint32_t* sources, uint32_t n_sources, double delta,
+                             double* edge_inout);
+
+/* Adjoint of the APSP sums with respect to the arc weights (lengths |w|), tie-split over shortest paths:
+ *   grad_sum_d[e]   += d(sum_{s in sources, t reached} d(s,t)) / d w_e
+ *   grad_sum_eff[e] += d(sum_{s, t reached, d != 0} 1/d(s,t)) / d w_e
+ * indexed like out_targets. The averages of compute_apsp_impl (AllPairsShortestPath.h:154-158) divide
+ * these sums by the (piecewise constant) number of reached pairs. sigma is recomputed per source batch;
+ * no predecessor DAG is stored. out_opt[n_sources] as in ng_brandes_accumulate. No reference counterpart. */
+ng_status ng_apsp_adjoint(ng_graph* g, const uint32_t* sources, uint32_t n_sources, double delta,
+                          double* grad_sum_d, double* grad_sum_eff, ng_sssp_summary* out_opt);
+
+/* Picks the bucket width for (lkind, mode) by timing candidate widths on `sample` sources and stores it
+ * in the graph (used whenever delta <= 0). mode: 0 = SSSP, 1 = Brandes. The reference takes a fixed
+ * integer --apsp-delta (neurograph.cpp:515); tuning is this build's extension. */
+ng_status ng_autotune_delta(ng_graph* g, ng_length_kind lkind, int mode, const uint32_t* sample, uint32_t n_sample,
+                            double* delta_out);
+
+/* Stats of the last successful call on `g`. */
+ng_status ng_graph_last_stats(const ng_graph* g, ng_call_stats* out);
+
+/* Cohort: many graphs of uneven size, all sources of all graphs, in one persistent launch captured in
+ * a hipGraph and replayed. The member graphs are borrowed (must outlive the cohort). No reference
+ * counterpart (the reference runs one graph per process). */
+ng_status ng_cohort_create(ng_context* ctx, ng_graph* const* graphs, uint32_t n_graphs, ng_cohort** out);
+void ng_cohort_destroy(ng_cohort* c);
+/* bc_out: concatenation of the graphs' betweenness vectors (sum of n_i doubles), overwritten;
+ * summaries_out: one summary per (graph, source) in graph order, optional. */
+ng_status ng_cohort_brandes(ng_cohort* c, double* bc_out, ng_sssp_summary* summaries_out, double* kernel_ms);
+
+const char* ng_last_error(void);
+
+/* Library version string, e.g. "ng_hip 0.1 gfx950". */
+const char* ng_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NEUROGRAPH_HIP_H */

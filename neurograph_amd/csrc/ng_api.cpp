@@ -1,0 +1,751 @@
+// C-ABI layer of libng_hip.so: contexts, device-resident graphs, the batched path driver.
+// See include/neurograph_hip.h for the contract of every entry point.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "ng_internal.h"
+
+using namespace ng;
+
+namespace {
+
+thread_local std::string g_error;
+
+ng_status fail(ng_status st, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return st;
+}
+
+#define NG_HIP(call)                                                                                  \
+    do {                                                                                              \
+        hipError_t e_ = (call);                                                                       \
+        if (e_ != hipSuccess) {                                                                       \
+            return fail(e_ == hipErrorOutOfMemory ? NG_ENOMEM : NG_EDEVICE, "%s failed: %s (%s:%d)", #call, \
+                        hipGetErrorString(e_), __FILE__, __LINE__);                                   \
+        }                                                                                             \
+    } while (0)
+
+template <class T>
+void dfree(T*& p) {
+    if (p) (void)hipFree(static_cast<void*>(p));
+    p = nullptr;
+}
+
+size_t dk_size(DKind dk) { return (dk == DKind::U32 || dk == DKind::F32) ? 4 : 8; }
+
+int dist_kind_of(DKind dk) {
+    switch (dk) {
+    case DKind::U32:
+    case DKind::U64: return NG_DIST_U64;
+    case DKind::F32: return NG_DIST_F32;
+    default: return NG_DIST_F64;
+    }
+}
+
+// Runs fn(begin, end) over [0, count) on up to hardware_concurrency threads.
+template <class Fn>
+void parallel_for(uint64_t count, Fn&& fn) {
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (count < 4096) nt = 1;
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nt; ++t) {
+        const uint64_t b = count * t / nt, e = count * (t + 1) / nt;
+        if (nt == 1) {
+            fn(b, e);
+        } else {
+            pool.emplace_back([b, e, &fn] { fn(b, e); });
+        }
+    }
+    for (auto& th : pool) th.join();
+}
+
+}  // namespace
+
+struct ng_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int cus = 0;
+};
+
+struct ng_graph {
+    ng_context* ctx = nullptr;
+    uint32_t n = 0;
+    uint64_t m = 0;
+    ng_weight_kind wk = NG_W_I32;
+    std::vector<uint64_t> off;
+    std::vector<uint32_t> tgt;
+    std::vector<unsigned char> w;  // raw weights
+    bool has_negative = false;
+    View views[2];                 // [0] |w|, [1] 1/|w|
+    Workspace ws;
+    DKind ws_dist_kind = DKind::U32;
+    bool ws_dist_valid = false;
+    ng_call_stats last{};
+};
+
+struct ng_cohort {
+    ng_context* ctx = nullptr;
+    std::vector<ng_graph*> graphs;
+};
+
+namespace {
+
+double weight_at(const ng_graph* g, uint64_t e) {
+    switch (g->wk) {
+    case NG_W_I32: return static_cast<double>(reinterpret_cast<const int32_t*>(g->w.data())[e]);
+    case NG_W_F32: return static_cast<double>(reinterpret_cast<const float*>(g->w.data())[e]);
+    default: return reinterpret_cast<const double*>(g->w.data())[e];
+    }
+}
+
+// Host-side construction of the device CSR for |w| (vi = 0) or 1/|w| (vi = 1).
+// Lengths follow DijkstraCommon::transform_weight (DijkstraCommon.h:36-54).
+template <class L>
+ng_status build_view_typed(ng_graph* g, int vi, DKind dk) {
+    View& v = g->views[vi];
+    const uint32_t n = g->n;
+    const uint64_t m = g->m;
+    std::vector<L> len(m);
+    const bool inverse = vi == 1;
+    parallel_for(m, [&](uint64_t b, uint64_t e) {
+        for (uint64_t a = b; a < e; ++a) {
+            if (g->wk == NG_W_I32) {
+                const long long aw = std::llabs(static_cast<long long>(reinterpret_cast<const int32_t*>(g->w.data())[a]));
+                if (inverse) {
+                    len[a] = aw == 0 ? std::numeric_limits<L>::infinity() : static_cast<L>(1.0 / static_cast<double>(aw));
+                } else {
+                    len[a] = static_cast<L>(aw);
+                }
+            } else if (g->wk == NG_W_F32) {
+                const float aw = std::fabs(reinterpret_cast<const float*>(g->w.data())[a]);
+                if (inverse) {
+                    len[a] = aw == 0.0f ? std::numeric_limits<L>::infinity() : static_cast<L>(1.0 / static_cast<double>(aw));
+                } else {
+                    len[a] = static_cast<L>(aw);
+                }
+            } else {
+                const double aw = std::fabs(reinterpret_cast<const double*>(g->w.data())[a]);
+                if (inverse) {
+                    len[a] = aw == 0.0 ? std::numeric_limits<L>::infinity() : static_cast<L>(1.0 / aw);
+                } else {
+                    len[a] = static_cast<L>(aw);
+                }
+            }
+        }
+    });
+    // sort every row by (length, target): light arcs become a row prefix
+    std::vector<uint32_t> perm(m), stgt(m);
+    std::vector<L> slen(m);
+    parallel_for(n, [&](uint64_t b, uint64_t e) {
+        std::vector<uint32_t> idx;
+        for (uint64_t u = b; u < e; ++u) {
+            const uint64_t r0 = g->off[u], r1 = g->off[u + 1];
+            idx.resize(r1 - r0);
+            std::iota(idx.begin(), idx.end(), static_cast<uint32_t>(r0));
+            std::stable_sort(idx.begin(), idx.end(), [&](uint32_t x, uint32_t y) {
+                if (len[x] != len[y]) return len[x] < len[y];
+                return g->tgt[x] < g->tgt[y];
+            });
+            for (uint64_t k = 0; k < idx.size(); ++k) {
+                perm[r0 + k] = idx[k];
+                stgt[r0 + k] = g->tgt[idx[k]];
+                slen[r0 + k] = len[idx[k]];
+            }
+        }
+    });
+    // statistics over finite lengths
+    double mn = std::numeric_limits<double>::infinity(), mx = 0.0, sum = 0.0;
+    uint64_t fin = 0;
+    bool zero = false;
+    for (uint64_t a = 0; a < m; ++a) {
+        const double x = static_cast<double>(slen[a]);
+        if (!std::isfinite(x)) continue;
+        ++fin;
+        sum += x;
+        mn = std::min(mn, x);
+        mx = std::max(mx, x);
+        if (x == 0.0) zero = true;
+    }
+    double q05 = 1.0;
+    if (fin) {
+        std::vector<double> sample;
+        const uint64_t step = std::max<uint64_t>(1, m / 65536);
+        for (uint64_t a = 0; a < m; a += step) {
+            const double x = static_cast<double>(slen[a]);
+            if (std::isfinite(x)) sample.push_back(x);
+        }
+        if (!sample.empty()) {
+            const size_t k = sample.size() / 20;
+            std::nth_element(sample.begin(), sample.begin() + k, sample.end());
+            q05 = sample[k];
+        }
+    }
+    std::vector<uint32_t> off32(static_cast<size_t>(n) + 1);
+    for (uint32_t u = 0; u <= n; ++u) off32[u] = static_cast<uint32_t>(g->off[u]);
+
+    NG_HIP(hipSetDevice(g->ctx->device));
+    NG_HIP(hipMalloc(&v.off, sizeof(uint32_t) * (static_cast<size_t>(n) + 1)));
+    NG_HIP(hipMalloc(&v.tgt, sizeof(uint32_t) * std::max<uint64_t>(m, 1)));
+    NG_HIP(hipMalloc(&v.len, sizeof(L) * std::max<uint64_t>(m, 1)));
+    NG_HIP(hipMalloc(&v.perm, sizeof(uint32_t) * std::max<uint64_t>(m, 1)));
+    NG_HIP(hipMalloc(&v.lend, sizeof(uint32_t) * std::max<uint32_t>(n, 1)));
+    NG_HIP(hipMemcpy(v.off, off32.data(), sizeof(uint32_t) * off32.size(), hipMemcpyHostToDevice));
+    if (m) {
+        NG_HIP(hipMemcpy(v.tgt, stgt.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice));
+        NG_HIP(hipMemcpy(v.len, slen.data(), sizeof(L) * m, hipMemcpyHostToDevice));
+        NG_HIP(hipMemcpy(v.perm, perm.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice));
+    }
+    v.dk = dk;
+    v.min_len = fin ? mn : 0.0;
+    v.max_len = mx;
+    v.mean_len = fin ? sum / static_cast<double>(fin) : 0.0;
+    v.q05_len = q05;
+    v.has_zero = zero;
+    v.lend_delta = -1.0;
+    v.built = true;
+    return NG_OK;
+}
+
+ng_status ensure_view(ng_graph* g, int vi) {
+    View& v = g->views[vi];
+    if (v.built) return NG_OK;
+    if (vi == 1) return build_view_typed<double>(g, 1, DKind::F64);
+    switch (g->wk) {
+    case NG_W_I32: {
+        uint64_t maxw = 0;
+        const int32_t* w = reinterpret_cast<const int32_t*>(g->w.data());
+        for (uint64_t a = 0; a < g->m; ++a) maxw = std::max<uint64_t>(maxw, static_cast<uint64_t>(std::llabs(static_cast<long long>(w[a]))));
+        // longest simple path <= (n-1) * max|w|; keep one value below the sentinel
+        const long double bound = static_cast<long double>(g->n) * static_cast<long double>(maxw);
+        if (bound < static_cast<long double>(0xFFFFFFF0u)) return build_view_typed<uint32_t>(g, 0, DKind::U32);
+        return build_view_typed<uint64_t>(g, 0, DKind::U64);
+    }
+    case NG_W_F32: return build_view_typed<float>(g, 0, DKind::F32);
+    default: return build_view_typed<double>(g, 0, DKind::F64);
+    }
+}
+
+void free_view(View& v) {
+    dfree(v.off);
+    dfree(v.tgt);
+    if (v.len) (void)hipFree(v.len);
+    v.len = nullptr;
+    dfree(v.perm);
+    dfree(v.src);
+    dfree(v.lend);
+    v.built = false;
+}
+
+void free_ws(Workspace& w) {
+    if (w.dist) (void)hipFree(w.dist);
+    w.dist = nullptr;
+    dfree(w.fstamp);
+    dfree(w.sstamp);
+    dfree(w.pstamp);
+    dfree(w.lists);
+    dfree(w.counters);
+    dfree(w.sigma);
+    dfree(w.delta1);
+    dfree(w.delta2);
+    dfree(w.indeg);
+    dfree(w.order);
+    dfree(w.layers);
+    dfree(w.bits);
+    dfree(w.log);
+    dfree(w.bc_slot);
+    if (w.summaries) (void)hipFree(w.summaries);
+    w.summaries = nullptr;
+    dfree(w.arcs_reached);
+    dfree(w.err);
+    dfree(w.sources);
+    dfree(w.bc_out);
+    dfree(w.grad1);
+    dfree(w.grad2);
+    w = Workspace{};
+}
+
+// Per-slot device bytes of the batched kernel for (n, m, mode).
+uint64_t slot_bytes(uint32_t n, uint64_t m, int mode, size_t dbytes, uint32_t logcap) {
+    uint64_t b = static_cast<uint64_t>(n) * (dbytes + 12 + 20) + 16;
+    if (mode != MODE_SSSP) {
+        b += static_cast<uint64_t>(n) * (8 + 8 + 4 + 4 + 4 + 8 + 8) + 4 + ((m + 31) / 32) * 4 + static_cast<uint64_t>(logcap) * 8;
+    }
+    return b;
+}
+
+uint32_t log_capacity(uint32_t n, uint64_t m) {
+    const uint64_t want = std::max<uint64_t>(static_cast<uint64_t>(n) * 8, 4096);
+    return static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>(m, 1), want));
+}
+
+// Makes sure the workspace holds `slots` slots of every array `mode` needs, with dist in kind dk.
+ng_status ensure_ws(ng_graph* g, uint32_t slots, int mode, DKind dk, uint32_t n_sources) {
+    Workspace& w = g->ws;
+    const hipStream_t st = g->ctx->stream;
+    const int need_bits = (mode == MODE_SSSP) ? 0 : (MODE_BC | MODE_ARC1 | MODE_ARC2);
+    const bool realloc = w.slots < slots || w.n != g->n || dk_size(dk) > w.dist_bytes || (need_bits & ~w.mode_bits) != 0;
+    if (realloc) {
+        free_ws(w);
+        const uint32_t n = g->n;
+        const uint64_t m = g->m;
+        const size_t ns = static_cast<size_t>(slots) * n;
+        w.slots = slots;
+        w.n = n;
+        w.m = m;
+        w.dist_bytes = 8;  // room for either element size
+        w.mode_bits = need_bits;
+        NG_HIP(hipMalloc(&w.dist, 8 * std::max<size_t>(ns, 1)));
+        NG_HIP(hipMalloc(&w.fstamp, 4 * std::max<size_t>(ns, 1)));
+        NG_HIP(hipMalloc(&w.sstamp, 4 * std::max<size_t>(ns, 1)));
+        NG_HIP(hipMalloc(&w.pstamp, 4 * std::max<size_t>(ns, 1)));
+        NG_HIP(hipMalloc(&w.lists, 20 * std::max<size_t>(ns, 1)));
+        NG_HIP(hipMalloc(&w.counters, 16 * static_cast<size_t>(slots)));
+        NG_HIP(launch_fill_u32(w.fstamp, ns, 0u, st));
+        NG_HIP(launch_fill_u32(w.sstamp, ns, 0u, st));
+        NG_HIP(launch_fill_u32(w.pstamp, ns, 0u, st));
+        NG_HIP(launch_fill_u32(w.counters, 4ull * slots, 1u, st));
+        if (need_bits) {
+            w.logcap = log_capacity(n, m);
+            w.words = (m + 31) / 32;
+            NG_HIP(hipMalloc(&w.sigma, 8 * std::max<size_t>(ns, 1)));
+            NG_HIP(hipMalloc(&w.delta1, 8 * std::max<size_t>(ns, 1)));
+            NG_HIP(hipMalloc(&w.delta2, 8 * std::max<size_t>(ns, 1)));
+            NG_HIP(hipMalloc(&w.indeg, 4 * std::max<size_t>(ns, 1)));
+            NG_HIP(hipMalloc(&w.order, 4 * std::max<size_t>(ns, 1)));
+            NG_HIP(hipMalloc(&w.layers, 4 * static_cast<size_t>(slots) * (n + 1)));
+            NG_HIP(hipMalloc(&w.bits, 4 * std::max<uint64_t>(w.words * slots, 1)));
+            NG_HIP(hipMalloc(&w.log, 8 * static_cast<size_t>(w.logcap) * slots));
+            NG_HIP(hipMalloc(&w.bc_slot, 8 * std::max<size_t>(ns, 1)));
+            NG_HIP(hipMalloc(&w.bc_out, 8 * std::max<uint32_t>(n, 1)));
+            NG_HIP(hipMalloc(&w.grad1, 8 * std::max<uint64_t>(m, 1)));
+            NG_HIP(hipMalloc(&w.grad2, 8 * std::max<uint64_t>(m, 1)));
+            NG_HIP(launch_fill_f64(w.sigma, ns, 0.0, st));
+            NG_HIP(launch_fill_f64(w.delta1, ns, 0.0, st));
+            NG_HIP(launch_fill_f64(w.delta2, ns, 0.0, st));
+            NG_HIP(launch_fill_u32(w.indeg, ns, 0u, st));
+            NG_HIP(launch_fill_u32(w.bits, w.words * slots, 0u, st));
+        }
+        NG_HIP(hipMalloc(&w.arcs_reached, 8));
+        NG_HIP(hipMalloc(&w.err, 4));
+        g->ws_dist_valid = false;
+    }
+    if (!g->ws_dist_valid || g->ws_dist_kind != dk) {
+        NG_HIP(launch_fill_dist(dk, w.dist, static_cast<uint64_t>(w.slots) * g->n, st));
+        g->ws_dist_kind = dk;
+        g->ws_dist_valid = true;
+    }
+    if (w.summaries_cap < n_sources) {
+        if (w.summaries) (void)hipFree(w.summaries);
+        w.summaries = nullptr;
+        NG_HIP(hipMalloc(&w.summaries, sizeof(ng_sssp_summary) * std::max<uint32_t>(n_sources, 1)));
+        w.summaries_cap = n_sources;
+    }
+    if (w.sources_cap < n_sources) {
+        dfree(w.sources);
+        NG_HIP(hipMalloc(&w.sources, sizeof(uint32_t) * std::max<uint32_t>(n_sources, 1)));
+        w.sources_cap = n_sources;
+    }
+    return NG_OK;
+}
+
+double pick_delta(const ng_graph* g, const View& v, int vi, int mode_idx, double delta) {
+    if (delta > 0.0) return delta;
+    (void)g;
+    (void)vi;
+    if (v.tuned_delta[mode_idx] > 0.0) return v.tuned_delta[mode_idx];
+    double d = v.q05_len;
+    if (!(d > 0.0) || !std::isfinite(d)) d = v.min_len > 0.0 ? v.min_len : 1.0;
+    if (v.dk == DKind::U32 || v.dk == DKind::U64) d = std::max(1.0, std::round(d));
+    return d;
+}
+
+struct RunSpec {
+    int vi;           // view index
+    int mode;         // MODE_*
+    double delta;
+    const uint32_t* sources;
+    uint32_t n_sources;
+    void* dist_out;   // host, optional
+};
+
+// Uploads sources, sizes the workspace, launches the batched kernel once, and synchronizes.
+ng_status run_batch(ng_graph* g, const RunSpec& rs, uint32_t* slots_used, double* used_delta) {
+    ng_context* ctx = g->ctx;
+    NG_HIP(hipSetDevice(ctx->device));
+    for (uint32_t i = 0; i < rs.n_sources; ++i) {
+        if (rs.sources[i] >= g->n) {
+            return fail(NG_EINVAL, "source %u is out of range, the graph has %u nodes (Dijkstra.h:66-67)", rs.sources[i], g->n);
+        }
+    }
+    ng_status st = ensure_view(g, rs.vi);
+    if (st != NG_OK) return st;
+    View& v = g->views[rs.vi];
+    const int mode_idx = rs.mode == MODE_SSSP ? 0 : 1;
+    const double delta = pick_delta(g, v, rs.vi, mode_idx, rs.delta);
+    if (used_delta) *used_delta = delta;
+    if (v.lend_delta != delta) {
+        NG_HIP(launch_light_end(v.dk, g->n, v.off, v.len, delta, v.lend, ctx->stream));
+        v.lend_delta = delta;
+    }
+    // slots: resident workgroups of this kernel, bounded by the sources and by device memory
+    const int bpc = batch_blocks_per_cu(v.dk, rs.mode);
+    uint64_t slots = static_cast<uint64_t>(bpc) * static_cast<uint64_t>(ctx->cus);
+    slots = std::min<uint64_t>(slots, std::max<uint32_t>(rs.n_sources, 1));
+    size_t free_b = 0, total_b = 0;
+    NG_HIP(hipMemGetInfo(&free_b, &total_b));
+    const uint64_t per = slot_bytes(g->n, g->m, rs.mode, 8, log_capacity(g->n, g->m));
+    // memory already held by this graph's workspace counts as available
+    const uint64_t held = static_cast<uint64_t>(g->ws.slots) * per;
+    const uint64_t budget = static_cast<uint64_t>(static_cast<double>(free_b + held) * 0.6);
+    if (per > 0) slots = std::min<uint64_t>(slots, std::max<uint64_t>(1, budget / per));
+    st = ensure_ws(g, static_cast<uint32_t>(slots), rs.mode, v.dk, rs.n_sources);
+    if (st != NG_OK) return st;
+    Workspace& w = g->ws;
+    const hipStream_t s = ctx->stream;
+    if (rs.n_sources) NG_HIP(hipMemcpyAsync(w.sources, rs.sources, sizeof(uint32_t) * rs.n_sources, hipMemcpyHostToDevice, s));
+    NG_HIP(hipMemsetAsync(w.arcs_reached, 0, 8, s));
+    NG_HIP(hipMemsetAsync(w.err, 0, 4, s));
+    if (rs.mode & MODE_BC) NG_HIP(launch_fill_f64(w.bc_slot, static_cast<uint64_t>(slots) * g->n, 0.0, s));
+    if (rs.mode & MODE_ARC1) NG_HIP(launch_fill_f64(w.grad1, g->m, 0.0, s));
+    if (rs.mode & MODE_ARC2) NG_HIP(launch_fill_f64(w.grad2, g->m, 0.0, s));
+    void* d_dist_out = nullptr;
+    const int dok = dist_kind_of(v.dk);
+    const size_t dob = (dok == NG_DIST_F32 ? 4 : 8);
+    if (rs.dist_out) NG_HIP(hipMalloc(&d_dist_out, dob * std::max<size_t>(static_cast<size_t>(rs.n_sources) * g->n, 1)));
+    LaunchArgs la{};
+    la.dk = v.dk;
+    la.mode = rs.mode;
+    la.n = g->n;
+    la.view = &v;
+    la.sources = w.sources;
+    la.n_sources = rs.n_sources;
+    la.delta = delta;
+    la.grid = static_cast<uint32_t>(slots);
+    la.ws = &w;
+    la.dist_out = d_dist_out;
+    la.dist_out_kind = dok;
+    float ms = 0.0f;
+    if (rs.n_sources) {
+        NG_HIP(hipEventRecord(ctx->ev0, s));
+        hipError_t le = launch_batch(la, s);
+        if (le != hipSuccess) {
+            if (d_dist_out) (void)hipFree(d_dist_out);
+            return fail(NG_EDEVICE, "batched kernel launch failed: %s", hipGetErrorString(le));
+        }
+        NG_HIP(hipEventRecord(ctx->ev1, s));
+    }
+    hipError_t se = hipStreamSynchronize(s);
+    if (se != hipSuccess) {
+        if (d_dist_out) (void)hipFree(d_dist_out);
+        return fail(NG_EDEVICE, "batched kernel failed: %s", hipGetErrorString(se));
+    }
+    if (rs.n_sources) NG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    uint32_t err = 0;
+    unsigned long long arcs = 0;
+    NG_HIP(hipMemcpy(&err, w.err, 4, hipMemcpyDeviceToHost));
+    NG_HIP(hipMemcpy(&arcs, w.arcs_reached, 8, hipMemcpyDeviceToHost));
+    if (rs.dist_out) {
+        NG_HIP(hipMemcpy(rs.dist_out, d_dist_out, dob * static_cast<size_t>(rs.n_sources) * g->n, hipMemcpyDeviceToHost));
+        NG_HIP(hipFree(d_dist_out));
+    }
+    if (err & ERR_KAHN) return fail(NG_EDEVICE, "tight-arc DAG has a cycle (zero-length arcs?)");
+    if (err & ERR_GUARD) return fail(NG_EDEVICE, "iteration guard hit in the batched kernel");
+    if (err & ERR_OVERFLOW) return fail(NG_EDEVICE, "queue overflow in the batched kernel");
+    g->last.kernel_ms = ms;
+    g->last.launches = rs.n_sources ? 1 : 0;
+    g->last.slots = static_cast<uint32_t>(slots);
+    g->last.delta = delta;
+    g->last.arcs_reached = arcs;
+    if (slots_used) *slots_used = static_cast<uint32_t>(slots);
+    return NG_OK;
+}
+
+ng_status check_graph(const ng_graph* g) {
+    if (!g || !g->ctx) return fail(NG_EINVAL, "null graph");
+    return NG_OK;
+}
+
+ng_status copy_summaries(ng_graph* g, uint32_t n_sources, ng_sssp_summary* out) {
+    if (!out || !n_sources) return NG_OK;
+    NG_HIP(hipMemcpy(out, g->ws.summaries, sizeof(ng_sssp_summary) * n_sources, hipMemcpyDeviceToHost));
+    return NG_OK;
+}
+
+// Brandes-family run on the |w| view; requires positive lengths.
+ng_status run_dependency(ng_graph* g, const uint32_t* sources, uint32_t n_sources, double delta, int mode, ng_sssp_summary* out_opt) {
+    ng_status st = ensure_view(g, 0);
+    if (st != NG_OK) return st;
+    if (g->views[0].has_zero) {
+        return fail(NG_EUNSUPPORTED, "zero-length arcs: Brandes needs non-zero arc weights (BrandesVariant.h:30-31)");
+    }
+    RunSpec rs{0, mode, delta, sources, n_sources, nullptr};
+    st = run_batch(g, rs, nullptr, nullptr);
+    if (st != NG_OK) return st;
+    return copy_summaries(g, n_sources, out_opt);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ng_last_error(void) { return g_error.c_str(); }
+
+// Internal: lets the I/O translation unit report through ng_last_error().
+void ng_set_error_message(const char* msg) { g_error = msg ? msg : ""; }
+
+const char* ng_version(void) { return "ng_hip 0.1 gfx950"; }
+
+ng_status ng_context_create(int device, ng_context** out) {
+    if (!out) return fail(NG_EINVAL, "null output pointer");
+    *out = nullptr;
+    int count = 0;
+    NG_HIP(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return fail(NG_EINVAL, "device %d out of range (%d devices)", device, count);
+    NG_HIP(hipSetDevice(device));
+    auto ctx = std::make_unique<ng_context>();
+    ctx->device = device;
+    hipDeviceProp_t prop;
+    NG_HIP(hipGetDeviceProperties(&prop, device));
+    ctx->cus = prop.multiProcessorCount;
+    NG_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    NG_HIP(hipEventCreate(&ctx->ev0));
+    NG_HIP(hipEventCreate(&ctx->ev1));
+    *out = ctx.release();
+    return NG_OK;
+}
+
+void ng_context_destroy(ng_context* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+ng_status ng_graph_create(ng_context* ctx, uint32_t n, uint64_t m, const uint64_t* out_offsets, const uint32_t* out_targets,
+                          const void* out_weights, ng_weight_kind wkind, ng_graph** out) {
+    if (!out) return fail(NG_EINVAL, "null output pointer");
+    *out = nullptr;
+    if (!ctx) return fail(NG_EINVAL, "null context");
+    if (!out_offsets) return fail(NG_EINVAL, "null offsets");
+    if (m && (!out_targets || !out_weights)) return fail(NG_EINVAL, "null targets or weights");
+    if (wkind != NG_W_I32 && wkind != NG_W_F32 && wkind != NG_W_F64) return fail(NG_EINVAL, "unknown weight kind %d", wkind);
+    if (m >= 0xFFFFFFFFull) return fail(NG_EUNSUPPORTED, "%llu arcs: arc ids are 32-bit (arc_id_type, Types.h:27)", (unsigned long long)m);
+    if (n == 0xFFFFFFFFu) return fail(NG_EUNSUPPORTED, "too many nodes");
+    if (out_offsets[0] != 0 || out_offsets[n] != m) return fail(NG_EINVAL, "offsets must start at 0 and end at m");
+    for (uint32_t u = 0; u < n; ++u) {
+        if (out_offsets[u] > out_offsets[u + 1]) return fail(NG_EINVAL, "offsets decrease at node %u", u);
+    }
+    for (uint64_t a = 0; a < m; ++a) {
+        if (out_targets[a] >= n) return fail(NG_EINVAL, "arc %llu targets node %u, the graph has %u nodes", (unsigned long long)a, out_targets[a], n);
+    }
+    auto g = std::make_unique<ng_graph>();
+    g->ctx = ctx;
+    g->n = n;
+    g->m = m;
+    g->wk = wkind;
+    g->off.assign(out_offsets, out_offsets + static_cast<size_t>(n) + 1);
+    g->tgt.assign(out_targets, out_targets + m);
+    const size_t wb = wkind == NG_W_F64 ? 8 : 4;
+    g->w.resize(wb * m);
+    if (m) std::memcpy(g->w.data(), out_weights, wb * m);
+    for (uint64_t a = 0; a < m; ++a) {
+        if (weight_at(g.get(), a) < 0.0) {
+            g->has_negative = true;
+            break;
+        }
+    }
+    ng_status st = ensure_view(g.get(), 0);
+    if (st != NG_OK) {
+        free_view(g->views[0]);
+        return st;
+    }
+    *out = g.release();
+    return NG_OK;
+}
+
+void ng_graph_destroy(ng_graph* g) {
+    if (!g) return;
+    (void)hipSetDevice(g->ctx->device);
+    (void)hipStreamSynchronize(g->ctx->stream);
+    free_view(g->views[0]);
+    free_view(g->views[1]);
+    free_ws(g->ws);
+    delete g;
+}
+
+ng_status ng_graph_dist_kind(const ng_graph* g, ng_length_kind lkind, ng_dist_kind* out) {
+    if (check_graph(g) != NG_OK) return NG_EINVAL;
+    if (!out) return fail(NG_EINVAL, "null output pointer");
+    if (lkind == NG_LEN_INVERSE) {
+        *out = NG_DIST_F64;
+        return NG_OK;
+    }
+    ng_status st = ensure_view(const_cast<ng_graph*>(g), 0);
+    if (st != NG_OK) return st;
+    *out = static_cast<ng_dist_kind>(dist_kind_of(g->views[0].dk));
+    return NG_OK;
+}
+
+ng_status ng_sssp_summaries(ng_graph* g, const uint32_t* sources, uint32_t n_sources, ng_length_kind lkind, double delta,
+                            ng_sssp_summary* out, void* dist_out) {
+    if (check_graph(g) != NG_OK) return NG_EINVAL;
+    if (n_sources && (!sources || !out)) return fail(NG_EINVAL, "null sources or output");
+    int vi = 0;
+    switch (lkind) {
+    case NG_LEN_ABS: vi = 0; break;
+    case NG_LEN_INVERSE: vi = 1; break;
+    case NG_LEN_RAW:
+        if (g->has_negative) return fail(NG_EINVAL, "negative arc weight: delta-stepping casts weights to distance_type (PushCheckVariant.h:82)");
+        vi = 0;
+        break;
+    default: return fail(NG_EINVAL, "unknown length kind %d", lkind);
+    }
+    RunSpec rs{vi, MODE_SSSP, delta, sources, n_sources, dist_out};
+    ng_status st = run_batch(g, rs, nullptr, nullptr);
+    if (st != NG_OK) return st;
+    return copy_summaries(g, n_sources, out);
+}
+
+ng_status ng_brandes_accumulate(ng_graph* g, const uint32_t* sources, uint32_t n_sources, double delta, double* bc_inout,
+                                ng_sssp_summary* out_opt) {
+    if (check_graph(g) != NG_OK) return NG_EINVAL;
+    if (n_sources && !sources) return fail(NG_EINVAL, "null sources");
+    if (!bc_inout) return fail(NG_EINVAL, "null betweenness buffer");
+    ng_status st = run_dependency(g, sources, n_sources, delta, MODE_BC, out_opt);
+    if (st != NG_OK) return st;
+    if (!n_sources) return NG_OK;
+    NG_HIP(launch_reduce_slots(g->ws.bc_slot, g->last.slots, g->n, g->ws.bc_out, g->ctx->stream));
+    std::vector<double> tmp(g->n);
+    NG_HIP(hipMemcpyAsync(tmp.data(), g->ws.bc_out, 8 * static_cast<size_t>(g->n), hipMemcpyDeviceToHost, g->ctx->stream));
+    NG_HIP(hipStreamSynchronize(g->ctx->stream));
+    for (uint32_t v = 0; v < g->n; ++v) bc_inout[v] += tmp[v];
+    return NG_OK;
+}
+
+ng_status ng_edge_dependency(ng_graph* g, const uint32_t* sources, uint32_t n_sources, double delta, double* edge_inout) {
+    if (check_graph(g) != NG_OK) return NG_EINVAL;
+    if (n_sources && !sources) return fail(NG_EINVAL, "null sources");
+    if (!edge_inout) return fail(NG_EINVAL, "null edge buffer");
+    ng_status st = run_dependency(g, sources, n_sources, delta, MODE_ARC1, nullptr);
+    if (st != NG_OK) return st;
+    if (!n_sources) return NG_OK;
+    std::vector<double> tmp(g->m);
+    if (g->m) NG_HIP(hipMemcpy(tmp.data(), g->ws.grad1, 8 * g->m, hipMemcpyDeviceToHost));
+    for (uint64_t a = 0; a < g->m; ++a) edge_inout[a] += tmp[a];
+    return NG_OK;
+}
+
+ng_status ng_apsp_adjoint(ng_graph* g, const uint32_t* sources, uint32_t n_sources, double delta, double* grad_sum_d,
+                          double* grad_sum_eff, ng_sssp_summary* out_opt) {
+    if (check_graph(g) != NG_OK) return NG_EINVAL;
+    if (n_sources && !sources) return fail(NG_EINVAL, "null sources");
+    if (!grad_sum_d || !grad_sum_eff) return fail(NG_EINVAL, "null gradient buffer");
+    ng_status st = run_dependency(g, sources, n_sources, delta, MODE_ARC1 | MODE_ARC2, out_opt);
+    if (st != NG_OK) return st;
+    if (!n_sources) return NG_OK;
+    std::vector<double> t1(g->m), t2(g->m);
+    if (g->m) {
+        NG_HIP(hipMemcpy(t1.data(), g->ws.grad1, 8 * g->m, hipMemcpyDeviceToHost));
+        NG_HIP(hipMemcpy(t2.data(), g->ws.grad2, 8 * g->m, hipMemcpyDeviceToHost));
+    }
+    // d|w|/dw = sign(w); the efficiency sum decreases with every length on a shortest path
+    for (uint64_t a = 0; a < g->m; ++a) {
+        const double w = weight_at(g, a);
+        const double sg = w > 0.0 ? 1.0 : (w < 0.0 ? -1.0 : 0.0);
+        grad_sum_d[a] += sg * t1[a];
+        grad_sum_eff[a] -= sg * t2[a];
+    }
+    return NG_OK;
+}
+
+ng_status ng_autotune_delta(ng_graph* g, ng_length_kind lkind, int mode, const uint32_t* sample, uint32_t n_sample,
+                            double* delta_out) {
+    if (check_graph(g) != NG_OK) return NG_EINVAL;
+    if (!sample || !n_sample) return fail(NG_EINVAL, "empty sample");
+    const int vi = lkind == NG_LEN_INVERSE ? 1 : 0;
+    ng_status st = ensure_view(g, vi);
+    if (st != NG_OK) return st;
+    View& v = g->views[vi];
+    if (mode != 0 && v.has_zero) return fail(NG_EUNSUPPORTED, "zero-length arcs");
+    const bool integer = v.dk == DKind::U32 || v.dk == DKind::U64;
+    // candidates: multiples of the 5% length quantile between the minimum length and the mean length
+    std::vector<double> cand;
+    const double base = v.q05_len > 0.0 ? v.q05_len : (v.min_len > 0.0 ? v.min_len : 1.0);
+    for (double f : {0.25, 0.5, 1.0, 2.0, 4.0, 8.0}) {
+        double d = base * f;
+        if (integer) d = std::max(1.0, std::round(d));
+        if (std::find(cand.begin(), cand.end(), d) == cand.end()) cand.push_back(d);
+    }
+    const int run_mode = mode == 0 ? MODE_SSSP : MODE_BC;
+    double best = cand.front(), best_ms = std::numeric_limits<double>::infinity();
+    for (double d : cand) {
+        double total = 0.0;
+        for (int rep = 0; rep < 2; ++rep) {
+            RunSpec rs{vi, run_mode, d, sample, n_sample, nullptr};
+            st = run_batch(g, rs, nullptr, nullptr);
+            if (st != NG_OK) return st;
+            if (rep == 1) total += g->last.kernel_ms;
+        }
+        if (total < best_ms) {
+            best_ms = total;
+            best = d;
+        }
+    }
+    v.tuned_delta[mode == 0 ? 0 : 1] = best;
+    if (delta_out) *delta_out = best;
+    return NG_OK;
+}
+
+ng_status ng_graph_last_stats(const ng_graph* g, ng_call_stats* out) {
+    if (check_graph(g) != NG_OK) return NG_EINVAL;
+    if (!out) return fail(NG_EINVAL, "null output pointer");
+    *out = g->last;
+    return NG_OK;
+}
+
+ng_status ng_cohort_create(ng_context* ctx, ng_graph* const* graphs, uint32_t n_graphs, ng_cohort** out) {
+    if (!out) return fail(NG_EINVAL, "null output pointer");
+    *out = nullptr;
+    if (!ctx || !graphs || !n_graphs) return fail(NG_EINVAL, "empty cohort");
+    auto c = std::make_unique<ng_cohort>();
+    c->ctx = ctx;
+    for (uint32_t i = 0; i < n_graphs; ++i) {
+        if (!graphs[i] || graphs[i]->ctx != ctx) return fail(NG_EINVAL, "graph %u is null or belongs to another context", i);
+        c->graphs.push_back(graphs[i]);
+    }
+    *out = c.release();
+    return NG_OK;
+}
+
+void ng_cohort_destroy(ng_cohort* c) { delete c; }
+
+ng_status ng_cohort_brandes(ng_cohort* c, double* bc_out, ng_sssp_summary* summaries_out, double* kernel_ms) {
+    (void)c;
+    (void)bc_out;
+    (void)summaries_out;
+    (void)kernel_ms;
+    return fail(NG_EUNSUPPORTED, "cohort launch not built yet");
+}
+
+}  // extern "C"

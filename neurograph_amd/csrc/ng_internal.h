@@ -1,0 +1,106 @@
+// Internal declarations shared by the C-ABI layer (ng_api.cpp) and the kernels (ng_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/neurograph_hip.h"
+
+namespace ng {
+
+// Device distance element type: lengths and distances are kept in one of these.
+enum class DKind : int { U32 = 0, U64 = 1, F32 = 2, F64 = 3 };
+
+// Kernel modes (bit flags). SSSP = distances + summaries only.
+enum : int { MODE_SSSP = 0, MODE_BC = 1, MODE_ARC1 = 2, MODE_ARC2 = 4 };
+
+// Device CSR of one length kind. Rows are sorted by (length, target) so that the arcs lighter than the
+// bucket width form a prefix of each row; perm maps a sorted position back to the caller's arc index.
+struct View {
+    bool built = false;
+    DKind dk = DKind::U32;
+    uint32_t* off = nullptr;   // [n+1]
+    uint32_t* tgt = nullptr;   // [m]
+    void* len = nullptr;       // [m], element type dk
+    uint32_t* perm = nullptr;  // [m]
+    uint32_t* src = nullptr;   // [m] source node of each sorted arc (used by the tight-arc log fallback)
+    double min_len = 0, max_len = 0, mean_len = 0, q05_len = 0;  // over finite lengths
+    bool has_zero = false;
+    double tuned_delta[2] = {0.0, 0.0};  // per mode (0 SSSP, 1 Brandes); 0 = not tuned
+    double lend_delta = -1.0;
+    uint32_t* lend = nullptr;  // [n] end of the light prefix of each row for lend_delta
+};
+
+// Per-slot scratch of the batched kernel. Sized for (slots, n, m); grown on demand, never shrunk.
+struct Workspace {
+    uint32_t slots = 0, n = 0;
+    uint64_t m = 0;
+    size_t dist_bytes = 0;  // element size of dist (4 or 8)
+    int mode_bits = 0;      // which optional arrays exist (MODE_* flags)
+    uint32_t logcap = 0;
+    void* dist = nullptr;       // [slots][n]
+    uint32_t* fstamp = nullptr; // [slots][n]
+    uint32_t* sstamp = nullptr; // [slots][n]
+    uint32_t* pstamp = nullptr; // [slots][n]
+    uint32_t* lists = nullptr;  // [slots][5][n]: frontier A/B, pile A/B, settled
+    uint32_t* counters = nullptr;  // [slots][4] persistent stamp generations
+    double* sigma = nullptr;    // [slots][n]
+    double* delta1 = nullptr;   // [slots][n]
+    double* delta2 = nullptr;   // [slots][n]
+    uint32_t* indeg = nullptr;  // [slots][n]
+    uint32_t* order = nullptr;  // [slots][n]
+    uint32_t* layers = nullptr; // [slots][n+1]
+    uint32_t* bits = nullptr;   // [slots][words]
+    uint64_t words = 0;
+    uint2* log = nullptr;       // [slots][logcap]
+    double* bc_slot = nullptr;  // [slots][n]
+    void* summaries = nullptr;  // [max_sources] ng_sssp_summary
+    uint32_t summaries_cap = 0;
+    uint64_t* arcs_reached = nullptr;  // [1]
+    uint32_t* err = nullptr;    // [1]
+    uint32_t* sources = nullptr;
+    uint32_t sources_cap = 0;
+    double* bc_out = nullptr;   // [n]
+    double* grad1 = nullptr;    // [m]
+    double* grad2 = nullptr;    // [m]
+};
+
+// Device error bits reported by the batched kernel.
+enum : uint32_t {
+    ERR_KAHN = 1u,
+    ERR_LOG_OVERFLOW = 2u,  // informational: the full-row tight scan was used
+    ERR_BUCKET = 4u,        // informational: a settle pass produced in-bucket work
+    ERR_GUARD = 8u,         // iteration guard hit: results of that source are incomplete
+    ERR_OVERFLOW = 16u      // a queue outgrew its capacity: results of that source are incomplete
+};
+
+struct LaunchArgs {
+    DKind dk;
+    int mode;
+    uint32_t n;
+    const View* view;
+    const uint32_t* sources;  // device
+    uint32_t n_sources;
+    double delta;
+    uint32_t grid;            // slots used
+    Workspace* ws;
+    void* dist_out;           // device, optional
+    int dist_out_kind;        // ng_dist_kind
+};
+
+// Launch wrappers (ng_kernels.hip). All enqueue on `stream`.
+hipError_t launch_batch(const LaunchArgs& a, hipStream_t stream);
+hipError_t launch_light_end(DKind dk, uint32_t n, const uint32_t* off, const void* len, double delta, uint32_t* lend,
+                            hipStream_t stream);
+hipError_t launch_reduce_slots(const double* bc_slot, uint32_t slots, uint32_t n, double* out, hipStream_t stream);
+hipError_t launch_fill_u32(uint32_t* p, uint64_t count, uint32_t value, hipStream_t stream);
+hipError_t launch_fill_f64(double* p, uint64_t count, double value, hipStream_t stream);
+hipError_t launch_fill_dist(DKind dk, void* p, uint64_t count, hipStream_t stream);  // fills with +infinity / max
+// Resident workgroups per CU of the batched kernel for (dk, mode).
+int batch_blocks_per_cu(DKind dk, int mode);
+constexpr int kBlock = 256;
+
+}  // namespace ng

@@ -1,0 +1,912 @@
+// HIP kernels for gfx950: multi-source shortest paths, Brandes sigma/delta sweeps and the path adjoint.
+//
+// One persistent launch serves a whole batch of sources. Workgroup b owns scratch slot b and runs the
+// sources b, b + grid, b + 2*grid, ... one after another (a static assignment, so the per-slot
+// betweenness partial sums, and therefore the results, do not depend on scheduling).
+//
+// Per source a workgroup runs delta-stepping (bucket width delta) with the light/heavy arc split:
+//   * rows are sorted by length, so the light arcs (length < delta) of a node are a row prefix;
+//   * inner phases relax only light arcs of the current frontier until the bucket [lo, hi) is settled;
+//   * one settle pass then scans each settled node's row exactly once with its final distance,
+//     relaxing heavy arcs (which can only land at or beyond hi) and, for betweenness, logging every arc
+//     whose candidate does not exceed the target's current distance: a superset of the tight arcs;
+//   * the next bucket starts at the smallest distance parked in the pile (nodes beyond hi).
+// Frontier tiles of 256 nodes are staged in LDS with an exclusive scan over their arc counts, so the
+// 256 lanes walk consecutive arcs of the concatenated rows (coalesced CSR reads) whatever the degree
+// skew; queue appends are wave-aggregated (ballot + popcount, one LDS atomic per wave).
+//
+// Betweenness: the logged candidates are verified against the final distances (tight arcs set a bit in a
+// per-slot arc bitmap and count the target's tight in-degree); Kahn layers over the tight DAG give sigma
+// (exact integer sums in double, so order-independent) and a layer order; the reverse layers then pull
+// delta(u) = sum over tight arcs (u,t) of sigma(u) * (1 + delta(t)) / sigma(t), the per-arc term of
+// BrandesVariant.h:149-154, clearing the bitmap on the way. Nothing is allocated per source: every
+// per-slot array is restored to its clean state by touching only the reached nodes.
+//
+// Reads of data that other lanes modify with atomics go through ldc() (an agent-scope relaxed atomic
+// load, which bypasses the CU's L1); phase boundaries are bsync() (agent fence + workgroup barrier).
+
+#include "ng_internal.h"
+
+#include <cfloat>
+#include <cstdint>
+
+namespace ng {
+namespace {
+
+constexpr uint32_t kWaves = kBlock / 64;
+
+// ---------------------------------------------------------------------------------------------------
+// distance types: distances are stored as their bit patterns so that atomic min on the unsigned bits is
+// the min on the value (non-negative IEEE values order like their bits; +inf is the largest).
+template <class D>
+struct DT;
+template <>
+struct DT<uint32_t> {
+    using B = uint32_t;
+    using L = uint32_t;
+    static constexpr bool kInt = true;
+    static constexpr B kInf = 0xFFFFFFFFu;
+    __device__ static B add(B a, L l) { return a + l; }
+    __device__ static B sat_add(B a, L l) {
+        const uint64_t s = static_cast<uint64_t>(a) + l;
+        return s >= kInf ? kInf : static_cast<B>(s);
+    }
+    __device__ static double val(B b) { return static_cast<double>(b); }
+    __device__ static bool light(L l, L delta) { return l < delta; }
+};
+template <>
+struct DT<uint64_t> {
+    using B = unsigned long long;
+    using L = unsigned long long;
+    static constexpr bool kInt = true;
+    static constexpr B kInf = ~0ull;
+    __device__ static B add(B a, L l) { return a + l; }
+    __device__ static B sat_add(B a, L l) { return a > kInf - l ? kInf : a + l; }
+    __device__ static double val(B b) { return static_cast<double>(b); }
+    __device__ static bool light(L l, L delta) { return l < delta; }
+};
+template <>
+struct DT<float> {
+    using B = uint32_t;
+    using L = float;
+    static constexpr bool kInt = false;
+    static constexpr B kInf = 0x7f800000u;
+    __device__ static B add(B a, L l) { return __float_as_uint(__uint_as_float(a) + l); }
+    __device__ static B sat_add(B a, L l) { return add(a, l); }
+    __device__ static double val(B b) { return static_cast<double>(__uint_as_float(b)); }
+    __device__ static bool light(L l, L delta) { return l < delta; }
+};
+template <>
+struct DT<double> {
+    using B = unsigned long long;
+    using L = double;
+    static constexpr bool kInt = false;
+    static constexpr B kInf = 0x7ff0000000000000ull;
+    __device__ static B add(B a, L l) {
+        return static_cast<B>(__double_as_longlong(__longlong_as_double(static_cast<long long>(a)) + l));
+    }
+    __device__ static B sat_add(B a, L l) { return add(a, l); }
+    __device__ static double val(B b) { return __longlong_as_double(static_cast<long long>(b)); }
+    __device__ static bool light(L l, L delta) { return l < delta; }
+};
+
+// ---------------------------------------------------------------------------------------------------
+// memory helpers
+
+template <class T>
+__device__ __forceinline__ T ldc(const T* p) {
+    return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ void stc(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void bsync() {
+    __threadfence();
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------------------------------
+// workgroup primitives (64-wide waves, kBlock lanes)
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= static_cast<uint32_t>(o)) x += y;
+    }
+    return x;
+}
+
+template <class T, class Op>
+__device__ __forceinline__ T wave_reduce(T x, Op op) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x = op(x, __shfl_xor(x, o, 64));
+    return x;
+}
+
+// Wave-aggregated append of `val` for lanes with `pred` into list[cnt++]; all active lanes must call it.
+// Appends at or beyond `cap` are dropped (the count still grows, so the caller can detect it).
+__device__ __forceinline__ void push(bool pred, uint32_t val, uint32_t* list, uint32_t* lds_cnt, uint32_t cap) {
+    const unsigned long long mask = __ballot(pred);
+    if (mask == 0ull) return;
+    const uint32_t lane = lane_id();
+    const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<long long>(mask)) - 1);
+    const uint32_t rank = static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(lds_cnt, static_cast<uint32_t>(__popcll(mask)));
+    base = __shfl(base, static_cast<int>(leader), 64);
+    if (pred && base + rank < cap) list[base + rank] = val;
+}
+
+template <class B>
+struct Lds {
+    uint32_t t_v[kBlock];
+    uint32_t t_beg[kBlock];
+    uint32_t t_off[kBlock];
+    B t_dv[kBlock];
+    uint32_t wave[kWaves];
+    uint32_t cnt[8];
+    B runmin;
+    unsigned long long red_u[3 * kWaves];
+    double red_d[3 * kWaves];
+};
+
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t* total, uint32_t* lds_wave) {
+    const uint32_t inc = wave_incl_scan(x);
+    const uint32_t w = threadIdx.x >> 6;
+    if (lane_id() == 63) lds_wave[w] = inc;
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kWaves; ++i) {
+        const uint32_t v = lds_wave[i];
+        if (i < w) base += v;
+        tot += v;
+    }
+    __syncthreads();
+    *total = tot;
+    return base + inc - x;
+}
+
+// Expands a list of nodes over per-node arc ranges. node_fn(v, dv&, beg&, cnt&) is called for each listed
+// node (by one lane); arc_fn(v, dv, e) is then called once per arc e in [beg, beg+cnt), with the arcs of
+// a 256-node tile spread over all lanes. Must be called by the whole workgroup with a uniform count.
+template <class B, class NodeFn, class ArcFn>
+__device__ __forceinline__ void expand(const uint32_t* list, uint32_t count, Lds<B>& S, NodeFn node_fn, ArcFn arc_fn) {
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t base = 0; base < count; base += kBlock) {
+        const uint32_t i = base + tid;
+        uint32_t v = 0, beg = 0, cnt = 0;
+        B dv = 0;
+        if (i < count) {
+            v = list[i];
+            node_fn(v, dv, beg, cnt);
+        }
+        uint32_t total;
+        const uint32_t ex = block_excl_scan(cnt, &total, S.wave);
+        S.t_v[tid] = v;
+        S.t_dv[tid] = dv;
+        S.t_beg[tid] = beg;
+        S.t_off[tid] = ex;
+        __syncthreads();
+        for (uint32_t k = tid; k < total; k += kBlock) {
+            uint32_t lo = 0, hi = kBlock;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (S.t_off[mid] <= k) lo = mid; else hi = mid;
+            }
+            arc_fn(S.t_v[lo], S.t_dv[lo], S.t_beg[lo] + (k - S.t_off[lo]));
+        }
+        __syncthreads();
+    }
+}
+
+template <class D>
+struct KArgs {
+    using B = typename DT<D>::B;
+    using L = typename DT<D>::L;
+    uint32_t n;
+    const uint32_t* off;
+    const uint32_t* tgt;
+    const L* len;
+    const uint32_t* lend;
+    const uint32_t* perm;
+    const uint32_t* sources;
+    uint32_t n_sources;
+    L delta;
+    B* dist;
+    uint32_t* fstamp;
+    uint32_t* sstamp;
+    uint32_t* pstamp;
+    uint32_t* lists;
+    uint32_t* counters;
+    double* sigma;
+    double* delta1;
+    double* delta2;
+    uint32_t* indeg;
+    uint32_t* order;
+    uint32_t* layers;
+    uint32_t* bits;
+    uint64_t words;
+    uint2* log;
+    uint32_t logcap;
+    double* bc_slot;
+    double* grad1;
+    double* grad2;
+    ng_sssp_summary* summaries;
+    unsigned long long* arcs_reached;
+    uint32_t* err;
+    void* dist_out;
+    int dist_out_kind;
+};
+
+template <class D>
+__device__ __forceinline__ void write_dist_out(const KArgs<D>& a, size_t idx, typename DT<D>::B b) {
+    using T = DT<D>;
+    const bool inf = (b == T::kInf);
+    switch (a.dist_out_kind) {
+    case NG_DIST_U64:
+        if constexpr (T::kInt) {
+            static_cast<unsigned long long*>(a.dist_out)[idx] = inf ? ~0ull : static_cast<unsigned long long>(b);
+        }
+        break;
+    case NG_DIST_F32:
+        static_cast<float*>(a.dist_out)[idx] = inf ? FLT_MAX : static_cast<float>(T::val(b));
+        break;
+    default:
+        static_cast<double*>(a.dist_out)[idx] = inf ? DBL_MAX : T::val(b);
+        break;
+    }
+}
+
+__device__ __forceinline__ uint32_t range_mask(uint32_t wi, uint32_t b, uint32_t e) {
+    // bits of word wi that belong to arcs [b, e)
+    const uint32_t w0 = wi << 5;
+    uint32_t m = 0xFFFFFFFFu;
+    if (b > w0) m &= 0xFFFFFFFFu << (b - w0);
+    if (e < w0 + 32) m &= (1u << (e - w0)) - 1u;
+    return m;
+}
+
+template <class D, int MODE>
+__global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
+    using T = DT<D>;
+    using B = typename T::B;
+    using L = typename T::L;
+    __shared__ Lds<B> S;
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t slot = blockIdx.x;
+    const uint32_t n = a.n;
+    const size_t so = static_cast<size_t>(slot) * n;
+    B* dist = a.dist + so;
+    uint32_t* fstamp = a.fstamp + so;
+    uint32_t* sstamp = a.sstamp + so;
+    uint32_t* pstamp = a.pstamp + so;
+    uint32_t* lists = a.lists + so * 5;
+    uint32_t* const Fa = lists;
+    uint32_t* const Fb = lists + n;
+    uint32_t* const Pa = lists + 2 * static_cast<size_t>(n);
+    uint32_t* const Pb = lists + 3 * static_cast<size_t>(n);
+    uint32_t* const Sl = lists + 4 * static_cast<size_t>(n);
+    double* sigma = nullptr;
+    double* delta1 = nullptr;
+    double* delta2 = nullptr;
+    uint32_t* indeg = nullptr;
+    uint32_t* order = nullptr;
+    uint32_t* layers = nullptr;
+    uint32_t* bits = nullptr;
+    uint2* log = nullptr;
+    double* bc_slot = nullptr;
+    if constexpr (MODE != MODE_SSSP) {
+        sigma = a.sigma + so;
+        delta1 = a.delta1 + so;
+        indeg = a.indeg + so;
+        order = a.order + so;
+        layers = a.layers + static_cast<size_t>(slot) * (n + 1);
+        bits = a.bits + static_cast<size_t>(slot) * a.words;
+        log = a.log + static_cast<size_t>(slot) * a.logcap;
+        if constexpr ((MODE & MODE_ARC2) != 0) delta2 = a.delta2 + so;
+        if constexpr ((MODE & MODE_BC) != 0) bc_slot = a.bc_slot + so;
+    }
+    uint32_t* ctr = a.counters + static_cast<size_t>(slot) * 4;
+    uint32_t phase = ctr[0], gen = ctr[1], pgen = ctr[2];
+    const L delta = a.delta;
+
+    for (uint32_t si = blockIdx.x; si < a.n_sources; si += gridDim.x) {
+        if (phase > 0xE0000000u || gen > 0xE0000000u || pgen > 0xE0000000u) {
+            for (uint32_t v = tid; v < n; v += kBlock) {
+                stc(&fstamp[v], 0u);
+                stc(&sstamp[v], 0u);
+                stc(&pstamp[v], 0u);
+            }
+            phase = gen = pgen = 1;
+            bsync();
+        }
+        ++gen;
+        ++pgen;
+        ++phase;
+        const uint32_t s = a.sources[si];
+        if (tid == 0) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) S.cnt[k] = 0;
+            S.runmin = T::kInf;
+            stc(&dist[s], B(0));
+            Fa[0] = s;
+        }
+        uint32_t* Fc = Fa;
+        uint32_t* Fn = Fb;
+        uint32_t* Pc = Pa;
+        uint32_t* Pn = Pb;
+        uint32_t nF = 1;
+        uint32_t settle_from = 0;
+        B hi = T::sat_add(B(0), delta);
+        // Uniform iteration guard: phases and bucket advances are bounded, so a runaway loop (a bug, or a
+        // pathological input) ends with ERR_GUARD instead of occupying the device.
+        uint32_t guard = 0;
+        const uint32_t guard_max = 64u * n + 100000u;
+        bool aborted = false;
+        auto clampn = [&](uint32_t c) -> uint32_t {
+            if (c > n) {
+                if (tid == 0) atomicOr(a.err, ERR_OVERFLOW);
+                aborted = true;
+                return 0u;
+            }
+            return c;
+        };
+        bsync();
+
+        // relaxation of arc e from a node at distance dv; new frontier entries are stamped `nph`
+        auto relax = [&](uint32_t e, B dv, uint32_t nph, bool log_it_if_close, uint32_t v) {
+            const uint32_t t = a.tgt[e];
+            const B nd = T::add(dv, a.len[e]);
+            const B cur = ldc(&dist[t]);
+            bool pf = false, pp = false;
+            if (nd < cur) {
+                const B old = atomicMin(&dist[t], nd);
+                if (nd < old) {
+                    if (nd < hi) {
+                        pf = atomicExch(&fstamp[t], nph) != nph;
+                    } else {
+                        atomicMin(&S.runmin, nd);
+                        pp = atomicExch(&pstamp[t], pgen) != pgen;
+                    }
+                }
+            }
+            push(pf, t, Fn, &S.cnt[0], n);
+            push(pp, t, Pc, &S.cnt[1], n);
+            if constexpr (MODE != MODE_SSSP) {
+                const bool lg = log_it_if_close && nd <= cur;
+                const unsigned long long mask = __ballot(lg);
+                if (mask != 0ull) {
+                    const uint32_t lane = lane_id();
+                    const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<long long>(mask)) - 1);
+                    const uint32_t rank = static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
+                    uint32_t base = 0;
+                    if (lane == leader) base = atomicAdd(&S.cnt[3], static_cast<uint32_t>(__popcll(mask)));
+                    base = __shfl(base, static_cast<int>(leader), 64);
+                    if (lg && base + rank < a.logcap) log[base + rank] = make_uint2(v, e);
+                }
+            }
+        };
+
+        while (true) {
+            // ---- inner phases: light arcs of the frontier until the bucket [lo, hi) is settled
+            while (nF > 0) {
+                if (++guard > guard_max) {
+                    if (tid == 0) atomicOr(a.err, ERR_GUARD);
+                    aborted = true;
+                    break;
+                }
+                const uint32_t nph = phase + 1;
+                expand(
+                    Fc, nF, S,
+                    [&](uint32_t v, B& dv, uint32_t& beg, uint32_t& cnt) {
+                        dv = ldc(&dist[v]);
+                        beg = a.off[v];
+                        cnt = a.lend[v] - beg;
+                    },
+                    [&](uint32_t v, B dv, uint32_t e) { relax(e, dv, nph, false, v); });
+                // first visit in this bucket -> settled list
+                for (uint32_t base = 0; base < nF; base += kBlock) {
+                    const uint32_t i = base + tid;
+                    bool fresh = false;
+                    uint32_t v = 0;
+                    if (i < nF) {
+                        v = Fc[i];
+                        fresh = atomicExch(&sstamp[v], gen) != gen;
+                    }
+                    push(fresh, v, Sl, &S.cnt[2], n);
+                }
+                bsync();
+                nF = clampn(S.cnt[0]);
+                uint32_t* tmp = Fc;
+                Fc = Fn;
+                Fn = tmp;
+                phase = nph;
+                bsync();
+                if (tid == 0) S.cnt[0] = 0;
+                bsync();
+            }
+            if (aborted) break;
+            // ---- settle pass: every settled node's row once, with its final distance
+            const uint32_t nS = clampn(S.cnt[2]);
+            if (aborted) break;
+            if (nS > settle_from) {
+                const uint32_t nph = phase + 1;
+                expand(
+                    Sl + settle_from, nS - settle_from, S,
+                    [&](uint32_t v, B& dv, uint32_t& beg, uint32_t& cnt) {
+                        dv = ldc(&dist[v]);
+                        beg = (MODE != MODE_SSSP) ? a.off[v] : a.lend[v];
+                        cnt = a.off[v + 1] - beg;
+                    },
+                    [&](uint32_t v, B dv, uint32_t e) { relax(e, dv, nph, true, v); });
+                bsync();
+                settle_from = nS;
+                nF = clampn(S.cnt[0]);
+                if (aborted) break;
+                if (nF > 0) {  // cannot happen with monotone rounding; handled for safety
+                    if (tid == 0) atomicOr(a.err, ERR_BUCKET);
+                    uint32_t* tmp = Fc;
+                    Fc = Fn;
+                    Fn = tmp;
+                    phase = nph;
+                    bsync();
+                    if (tid == 0) S.cnt[0] = 0;
+                    bsync();
+                    continue;
+                }
+            }
+            const uint32_t nP = clampn(S.cnt[1]);
+            if (aborted || nP == 0) break;
+            if (++guard > guard_max) {
+                if (tid == 0) atomicOr(a.err, ERR_GUARD);
+                aborted = true;
+                break;
+            }
+            // ---- next bucket: [max(pile minimum, hi), +delta)
+            const B rm = S.runmin;
+            const B lo = rm > hi ? rm : hi;
+            hi = T::sat_add(lo, delta);
+            // lo + delta may round back to lo for real-valued distances: the window must not be empty,
+            // or the pile would never drain. The next representable value is bits + 1.
+            if (hi <= lo && lo != T::kInf) hi = lo + 1;
+            ++pgen;
+            const uint32_t nph = phase + 1;
+            bsync();
+            if (tid == 0) {
+                S.runmin = T::kInf;
+                S.cnt[0] = 0;
+                S.cnt[1] = 0;
+            }
+            bsync();
+            for (uint32_t base = 0; base < nP; base += kBlock) {
+                const uint32_t i = base + tid;
+                bool tf = false, tp = false;
+                uint32_t t = 0;
+                if (i < nP) {
+                    t = Pc[i];
+                    if (ldc(&sstamp[t]) != gen) {
+                        const B d = ldc(&dist[t]);
+                        if (d < hi) {
+                            tf = atomicExch(&fstamp[t], nph) != nph;
+                        } else {
+                            atomicMin(&S.runmin, d);
+                            tp = atomicExch(&pstamp[t], pgen) != pgen;
+                        }
+                    }
+                }
+                push(tf, t, Fn, &S.cnt[0], n);
+                push(tp, t, Pn, &S.cnt[1], n);
+            }
+            bsync();
+            nF = clampn(S.cnt[0]);
+            if (aborted) break;
+            {
+                uint32_t* tmp = Fc;
+                Fc = Fn;
+                Fn = tmp;
+                tmp = Pc;
+                Pc = Pn;
+                Pn = tmp;
+            }
+            phase = nph;
+            bsync();
+            if (tid == 0) S.cnt[0] = 0;
+            bsync();
+        }
+
+        const uint32_t nS = S.cnt[2] > n ? n : S.cnt[2];
+        // ---- per-source summary (SsspResult.h:43-64) over the reached nodes
+        {
+            unsigned long long su = 0, mx = 0, arcs = 0;
+            double sd = 0.0, se = 0.0;
+            for (uint32_t i = tid; i < nS; i += kBlock) {
+                const uint32_t v = Sl[i];
+                const B d = ldc(&dist[v]);
+                if constexpr (T::kInt) su += static_cast<unsigned long long>(d);
+                else sd += T::val(d);
+                if (d != B(0)) se += 1.0 / T::val(d);
+                mx = (static_cast<unsigned long long>(d) > mx) ? static_cast<unsigned long long>(d) : mx;
+                arcs += a.off[v + 1] - a.off[v];
+            }
+            auto addu = [](unsigned long long x, unsigned long long y) { return x + y; };
+            auto maxu = [](unsigned long long x, unsigned long long y) { return x > y ? x : y; };
+            auto addd = [](double x, double y) { return x + y; };
+            su = wave_reduce(su, addu);
+            mx = wave_reduce(mx, maxu);
+            arcs = wave_reduce(arcs, addu);
+            sd = wave_reduce(sd, addd);
+            se = wave_reduce(se, addd);
+            const uint32_t w = tid >> 6;
+            if (lane_id() == 0) {
+                S.red_u[w] = su;
+                S.red_u[kWaves + w] = mx;
+                S.red_u[2 * kWaves + w] = arcs;
+                S.red_d[w] = sd;
+                S.red_d[kWaves + w] = se;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                unsigned long long tsu = 0, tmx = 0, tar = 0;
+                double tsd = 0.0, tse = 0.0;
+                for (uint32_t k = 0; k < kWaves; ++k) {
+                    tsu += S.red_u[k];
+                    tmx = S.red_u[kWaves + k] > tmx ? S.red_u[kWaves + k] : tmx;
+                    tar += S.red_u[2 * kWaves + k];
+                    tsd += S.red_d[k];
+                    tse += S.red_d[kWaves + k];
+                }
+                ng_sssp_summary r;
+                if constexpr (T::kInt) {
+                    r.sum_d = static_cast<double>(tsu);
+                    r.sum_d_u64 = tsu;
+                    r.max_d = static_cast<double>(tmx);
+                    r.max_d_u64 = tmx;
+                } else {
+                    r.sum_d = tsd;
+                    r.sum_d_u64 = 0;
+                    r.max_d = T::val(static_cast<B>(tmx));
+                    r.max_d_u64 = 0;
+                }
+                r.sum_eff = tse;
+                r.n_unreached = static_cast<uint64_t>(n) - nS;
+                a.summaries[si] = r;
+                atomicAdd(a.arcs_reached, tar);
+            }
+        }
+
+        if constexpr (MODE != MODE_SSSP) if (!aborted) {
+            // ---- tight arcs: verify the logged candidates against the final distances
+            const uint32_t nlog = S.cnt[3];
+            auto mark = [&](uint32_t u, uint32_t e) {
+                const uint32_t t = a.tgt[e];
+                const B c = T::add(ldc(&dist[u]), a.len[e]);
+                if (c == ldc(&dist[t])) {
+                    atomicOr(&bits[e >> 5], 1u << (e & 31));
+                    atomicAdd(&indeg[t], 1u);
+                }
+            };
+            if (nlog <= a.logcap) {
+                for (uint32_t i = tid; i < nlog; i += kBlock) {
+                    const uint2 r = log[i];
+                    mark(r.x, r.y);
+                }
+            } else {
+                if (tid == 0) atomicOr(a.err, ERR_LOG_OVERFLOW);
+                expand(
+                    Sl, nS, S,
+                    [&](uint32_t v, B& dv, uint32_t& beg, uint32_t& cnt) {
+                        dv = 0;
+                        beg = a.off[v];
+                        cnt = a.off[v + 1] - beg;
+                    },
+                    [&](uint32_t v, B, uint32_t e) { mark(v, e); });
+            }
+            bsync();
+            // ---- Kahn layers over the tight DAG: sigma and a topological order
+            if (tid == 0) {
+                order[0] = s;
+                stc(&sigma[s], 1.0);
+                S.cnt[4] = 1;
+            }
+            bsync();
+            uint32_t lb = 0, le = 1, nl = 0;
+            while (lb < le && nl < n) {
+                if (tid == 0) layers[nl] = lb;
+                ++nl;
+                for (uint32_t i = lb + tid; i < le; i += kBlock) {
+                    const uint32_t u = order[i];
+                    const double su = ldc(&sigma[u]);
+                    const uint32_t b = a.off[u], e_end = a.off[u + 1];
+                    if (b < e_end) {
+                        for (uint32_t wi = b >> 5; wi <= (e_end - 1) >> 5; ++wi) {
+                            uint32_t w = ldc(&bits[wi]) & range_mask(wi, b, e_end);
+                            while (w) {
+                                const uint32_t e = (wi << 5) + static_cast<uint32_t>(__ffs(w) - 1);
+                                w &= w - 1;
+                                const uint32_t t = a.tgt[e];
+                                unsafeAtomicAdd(&sigma[t], su);
+                                if (atomicSub(&indeg[t], 1u) == 1u) {
+                                    const uint32_t pos = atomicAdd(&S.cnt[4], 1u);
+                                    if (pos < n) order[pos] = t;
+                                }
+                            }
+                        }
+                    }
+                }
+                bsync();
+                lb = le;
+                le = S.cnt[4] > n ? n : S.cnt[4];
+                bsync();
+            }
+            if (tid == 0) layers[nl] = le;
+            if (le != nS) {  // a zero-length tight cycle kept some nodes from being released
+                if (tid == 0) atomicOr(a.err, ERR_KAHN);
+                for (uint32_t i = tid; i < nS; i += kBlock) {
+                    const uint32_t u = Sl[i];
+                    stc(&indeg[u], 0u);
+                    const uint32_t b = a.off[u], e_end = a.off[u + 1];
+                    if (b < e_end)
+                        for (uint32_t wi = b >> 5; wi <= (e_end - 1) >> 5; ++wi) atomicAnd(&bits[wi], ~range_mask(wi, b, e_end));
+                }
+            }
+            bsync();
+            // ---- reverse layers: dependencies, per-node and per-arc accumulation, bitmap cleared
+            for (int ly = static_cast<int>(nl) - 1; ly >= 0; --ly) {
+                const uint32_t l0 = layers[ly], l1 = layers[ly + 1];
+                for (uint32_t i = l0 + tid; i < l1; i += kBlock) {
+                    const uint32_t u = order[i];
+                    const double su = ldc(&sigma[u]);
+                    double acc1 = 0.0, acc2 = 0.0;
+                    const uint32_t b = a.off[u], e_end = a.off[u + 1];
+                    if (b < e_end) {
+                        for (uint32_t wi = b >> 5; wi <= (e_end - 1) >> 5; ++wi) {
+                            uint32_t w = ldc(&bits[wi]) & range_mask(wi, b, e_end);
+                            if (w == 0u) continue;
+                            atomicAnd(&bits[wi], ~w);
+                            while (w) {
+                                const uint32_t e = (wi << 5) + static_cast<uint32_t>(__ffs(w) - 1);
+                                w &= w - 1;
+                                const uint32_t t = a.tgt[e];
+                                const double st = ldc(&sigma[t]);
+                                const double term1 = su * ((1.0 + ldc(&delta1[t])) / st);
+                                acc1 += term1;
+                                if constexpr ((MODE & MODE_ARC1) != 0) unsafeAtomicAdd(&a.grad1[a.perm[e]], term1);
+                                if constexpr ((MODE & MODE_ARC2) != 0) {
+                                    const double dt = T::val(ldc(&dist[t]));
+                                    const double term2 = su * ((1.0 / (dt * dt) + ldc(&delta2[t])) / st);
+                                    acc2 += term2;
+                                    unsafeAtomicAdd(&a.grad2[a.perm[e]], term2);
+                                }
+                            }
+                        }
+                    }
+                    if (ly > 0) {
+                        stc(&delta1[u], acc1);
+                        if constexpr ((MODE & MODE_ARC2) != 0) stc(&delta2[u], acc2);
+                        if constexpr ((MODE & MODE_BC) != 0) bc_slot[u] += acc1;
+                    }
+                }
+                bsync();
+            }
+        }
+
+        if (a.dist_out != nullptr) {
+            for (uint32_t v = tid; v < n; v += kBlock)
+                write_dist_out(a, static_cast<size_t>(si) * n + v, ldc(&dist[v]));
+        }
+        // ---- restore the slot: only reached nodes were touched (after an abort, frontier and pile nodes
+        // may hold distances too, so the whole array is reset)
+        if (aborted) {
+            for (uint32_t v = tid; v < n; v += kBlock) stc(&dist[v], T::kInf);
+        }
+        for (uint32_t i = tid; i < nS; i += kBlock) {
+            const uint32_t v = Sl[i];
+            stc(&dist[v], T::kInf);
+            if constexpr (MODE != MODE_SSSP) {
+                stc(&sigma[v], 0.0);
+                stc(&delta1[v], 0.0);
+                if constexpr ((MODE & MODE_ARC2) != 0) stc(&delta2[v], 0.0);
+            }
+        }
+        bsync();
+    }
+    if (tid == 0) {
+        ctr[0] = phase;
+        ctr[1] = gen;
+        ctr[2] = pgen;
+    }
+}
+
+// end of the light prefix of each (length-sorted) row
+template <class L>
+__global__ void light_end_kernel(uint32_t n, const uint32_t* off, const L* len, L delta, uint32_t* lend) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    uint32_t lo = off[v], hi = off[v + 1];
+    while (lo < hi) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        if (len[mid] < delta) lo = mid + 1; else hi = mid;
+    }
+    lend[v] = lo;
+}
+
+// out[v] = sum over slots of part[slot][v], in slot order (deterministic)
+__global__ void reduce_slots_kernel(const double* part, uint32_t slots, uint32_t n, double* out) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    double acc = 0.0;
+    for (uint32_t k = 0; k < slots; ++k) acc += part[static_cast<size_t>(k) * n + v];
+    out[v] = acc;
+}
+
+template <class T>
+__global__ void fill_kernel(T* p, uint64_t count, T value) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) p[i] = value;
+}
+
+template <class T>
+hipError_t fill(T* p, uint64_t count, T value, hipStream_t stream) {
+    if (count == 0) return hipSuccess;
+    const uint64_t blocks64 = (count + 255) / 256;
+    const uint32_t blocks = static_cast<uint32_t>(blocks64 < 65536 ? blocks64 : 65536);
+    hipLaunchKernelGGL(fill_kernel<T>, dim3(blocks), dim3(256), 0, stream, p, count, value);
+    return hipGetLastError();
+}
+
+template <class D, int MODE>
+hipError_t launch_typed(const LaunchArgs& la, hipStream_t stream) {
+    using T = DT<D>;
+    using B = typename T::B;
+    using L = typename T::L;
+    KArgs<D> k{};
+    const View& vw = *la.view;
+    Workspace& ws = *la.ws;
+    k.n = la.n;
+    k.off = vw.off;
+    k.tgt = vw.tgt;
+    k.len = static_cast<const L*>(vw.len);
+    k.lend = vw.lend;
+    k.perm = vw.perm;
+    k.sources = la.sources;
+    k.n_sources = la.n_sources;
+    if constexpr (T::kInt) {
+        double d = la.delta < 1.0 ? 1.0 : la.delta;
+        k.delta = static_cast<L>(d + 0.5);
+    } else {
+        k.delta = static_cast<L>(la.delta);
+    }
+    k.dist = static_cast<B*>(ws.dist);
+    k.fstamp = ws.fstamp;
+    k.sstamp = ws.sstamp;
+    k.pstamp = ws.pstamp;
+    k.lists = ws.lists;
+    k.counters = ws.counters;
+    k.sigma = ws.sigma;
+    k.delta1 = ws.delta1;
+    k.delta2 = ws.delta2;
+    k.indeg = ws.indeg;
+    k.order = ws.order;
+    k.layers = ws.layers;
+    k.bits = ws.bits;
+    k.words = ws.words;
+    k.log = ws.log;
+    k.logcap = ws.logcap;
+    k.bc_slot = ws.bc_slot;
+    k.grad1 = ws.grad1;
+    k.grad2 = ws.grad2;
+    k.summaries = static_cast<ng_sssp_summary*>(ws.summaries);
+    k.arcs_reached = reinterpret_cast<unsigned long long*>(ws.arcs_reached);
+    k.err = ws.err;
+    k.dist_out = la.dist_out;
+    k.dist_out_kind = la.dist_out_kind;
+    hipLaunchKernelGGL((batch_kernel<D, MODE>), dim3(la.grid), dim3(kBlock), 0, stream, k);
+    return hipGetLastError();
+}
+
+template <class D>
+hipError_t launch_mode(const LaunchArgs& a, hipStream_t stream) {
+    switch (a.mode) {
+    case MODE_SSSP: return launch_typed<D, MODE_SSSP>(a, stream);
+    case MODE_BC: return launch_typed<D, MODE_BC>(a, stream);
+    case MODE_ARC1: return launch_typed<D, MODE_ARC1>(a, stream);
+    case MODE_ARC1 | MODE_ARC2: return launch_typed<D, MODE_ARC1 | MODE_ARC2>(a, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <class D>
+int blocks_per_cu_mode(int mode) {
+    int nb = 0;
+    hipError_t e = hipErrorInvalidValue;
+    switch (mode) {
+    case MODE_SSSP: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<D, MODE_SSSP>, kBlock, 0); break;
+    case MODE_BC: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<D, MODE_BC>, kBlock, 0); break;
+    case MODE_ARC1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<D, MODE_ARC1>, kBlock, 0); break;
+    default:
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<D, MODE_ARC1 | MODE_ARC2>, kBlock, 0);
+        break;
+    }
+    return (e == hipSuccess && nb > 0) ? nb : 1;
+}
+
+}  // namespace
+
+hipError_t launch_batch(const LaunchArgs& a, hipStream_t stream) {
+    switch (a.dk) {
+    case DKind::U32: return launch_mode<uint32_t>(a, stream);
+    case DKind::U64: return launch_mode<uint64_t>(a, stream);
+    case DKind::F32: return launch_mode<float>(a, stream);
+    case DKind::F64: return launch_mode<double>(a, stream);
+    }
+    return hipErrorInvalidValue;
+}
+
+int batch_blocks_per_cu(DKind dk, int mode) {
+    switch (dk) {
+    case DKind::U32: return blocks_per_cu_mode<uint32_t>(mode);
+    case DKind::U64: return blocks_per_cu_mode<uint64_t>(mode);
+    case DKind::F32: return blocks_per_cu_mode<float>(mode);
+    case DKind::F64: return blocks_per_cu_mode<double>(mode);
+    }
+    return 1;
+}
+
+hipError_t launch_light_end(DKind dk, uint32_t n, const uint32_t* off, const void* len, double delta, uint32_t* lend,
+                            hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const dim3 grid((n + 255) / 256), block(256);
+    switch (dk) {
+    case DKind::U32: {
+        const uint32_t d = static_cast<uint32_t>((delta < 1.0 ? 1.0 : delta) + 0.5);
+        hipLaunchKernelGGL(light_end_kernel<uint32_t>, grid, block, 0, stream, n, off, static_cast<const uint32_t*>(len), d, lend);
+        break;
+    }
+    case DKind::U64: {
+        const unsigned long long d = static_cast<unsigned long long>((delta < 1.0 ? 1.0 : delta) + 0.5);
+        hipLaunchKernelGGL(light_end_kernel<unsigned long long>, grid, block, 0, stream, n, off,
+                           static_cast<const unsigned long long*>(len), d, lend);
+        break;
+    }
+    case DKind::F32:
+        hipLaunchKernelGGL(light_end_kernel<float>, grid, block, 0, stream, n, off, static_cast<const float*>(len),
+                           static_cast<float>(delta), lend);
+        break;
+    case DKind::F64:
+        hipLaunchKernelGGL(light_end_kernel<double>, grid, block, 0, stream, n, off, static_cast<const double*>(len), delta,
+                           lend);
+        break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_slots(const double* bc_slot, uint32_t slots, uint32_t n, double* out, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(reduce_slots_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, bc_slot, slots, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint64_t count, uint32_t value, hipStream_t stream) {
+    return fill<uint32_t>(p, count, value, stream);
+}
+hipError_t launch_fill_f64(double* p, uint64_t count, double value, hipStream_t stream) {
+    return fill<double>(p, count, value, stream);
+}
+hipError_t launch_fill_dist(DKind dk, void* p, uint64_t count, hipStream_t stream) {
+    switch (dk) {
+    case DKind::U32: return fill<uint32_t>(static_cast<uint32_t*>(p), count, 0xFFFFFFFFu, stream);
+    case DKind::F32: return fill<uint32_t>(static_cast<uint32_t*>(p), count, 0x7f800000u, stream);
+    case DKind::U64: return fill<unsigned long long>(static_cast<unsigned long long*>(p), count, ~0ull, stream);
+    case DKind::F64: return fill<unsigned long long>(static_cast<unsigned long long*>(p), count, 0x7ff0000000000000ull, stream);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace ng

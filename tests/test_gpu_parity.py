@@ -1,0 +1,231 @@
+"""GPU parity: the HIP path (through the C ABI of libng_hip.so) against the reference's own known-answer
+values and against the CPU oracle on the same seeded inputs.
+
+Bars: integer distances, sums, maxima, unreachable counts, reach histograms and sigma-derived exact
+quantities are bit-exact; real-valued distances are bit-exact too (same single IEEE add per relaxation,
+the fixpoint min over the same candidates); order-dependent double sums (efficiency sums, betweenness,
+dependencies, gradients) are compared with a relative tolerance stated per test (the reference's own
+tests use 1e-6 absolute on these).
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from oracle import oracle as orc
+from tests.conftest import csr_from_fixture
+
+DELTAS = [0.0, 1, 3, 4, 10, 10000]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import neurograph_amd as ng
+
+    return ng.Context(0)
+
+
+def dev_graph(ctx, c):
+    import neurograph_amd as ng
+
+    return ng.Graph(ctx, ng.HostGraph(c.n, c.off, c.tgt, c.w))
+
+
+def rand_graph(n, m, seed, wdtype=np.int32, wlo=1, whi=10, parallel=True, self_loops=True):
+    rng = np.random.default_rng(seed)
+    src = rng.integers(0, n, size=m)
+    # skewed out-degrees: a few hubs
+    hubs = rng.integers(0, n, size=max(1, n // 50))
+    src[: m // 4] = rng.choice(hubs, size=m // 4)
+    dst = rng.integers(0, n, size=m)
+    if not self_loops:
+        keep = src != dst
+        src, dst = src[keep], dst[keep]
+    if not parallel:
+        _, idx = np.unique(src * n + dst, return_index=True)
+        src, dst = src[idx], dst[idx]
+    if np.issubdtype(wdtype, np.integer):
+        w = rng.integers(wlo, whi + 1, size=len(src)).astype(wdtype)
+    else:
+        w = np.exp(rng.normal(0.0, 0.5, size=len(src))).astype(wdtype)
+    return orc.Csr.from_edges(n, src, dst, w, dtype=wdtype)
+
+
+FIXTURES = ["standard", "standard_uu", "full", "negative", "four", "four_uu", "full36", "seven", "seven_uu", "full63"]
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_fixture_per_source_tables(ctx, reference_fixtures, name):
+    fx = reference_fixtures["fixtures"][name]
+    g = dev_graph(ctx, csr_from_fixture(fx))
+    rows = fx["per_source"]["rows"]
+    for delta in DELTAS:
+        s = g.sssp(None, delta=delta)
+        for i, (sd, mx, eff, un) in enumerate(rows):
+            assert int(s["sum_d_u64"][i]) == sd, (name, delta, i)
+            assert int(s["max_d_u64"][i]) == mx, (name, delta, i)
+            assert int(s["n_unreached"][i]) == un, (name, delta, i)
+            assert s["sum_eff"][i] == pytest.approx(eff, abs=fx["per_source"]["eff_tol"]), (name, delta, i)
+
+
+@pytest.mark.parametrize("name", [f for f in FIXTURES if f != "negative"] + ["disconnected"])
+def test_fixture_apsp_results(ctx, reference_fixtures, name):
+    from neurograph_amd import NG_LEN_ABS, NG_LEN_INVERSE
+    from neurograph_amd.metrics import apsp
+
+    fx = reference_fixtures["fixtures"][name]
+    g = dev_graph(ctx, csr_from_fixture(fx))
+    for key, lk in (("apsp", NG_LEN_ABS), ("apsp_inverse", NG_LEN_INVERSE)):
+        if key not in fx:
+            continue
+        exp = fx[key]
+        got = apsp(g, lk)
+        if exp["spl_exact"] and lk == NG_LEN_ABS:
+            assert got.average_shortest_path_length == exp["avg_spl"]
+        else:
+            assert got.average_shortest_path_length == pytest.approx(exp["avg_spl"], abs=1e-6)
+        assert got.average_efficiency == pytest.approx(exp["avg_eff"], abs=1e-6)
+        assert got.diameter == pytest.approx(exp["diameter"], abs=1e-6)
+        assert got.number_disconnected_pairs == exp["n_disconnected"]
+        assert got.cluster_sizes == {int(k): v for k, v in exp["cluster_sizes"].items()}
+
+
+@pytest.mark.parametrize("name", [f for f in FIXTURES if f != "negative"])
+def test_fixture_brandes(ctx, reference_fixtures, name):
+    fx = reference_fixtures["fixtures"][name]
+    c = csr_from_fixture(fx)
+    g = dev_graph(ctx, c)
+    ref = orc.brandes(c)
+    for delta in (0.0, 1, 4, 10000):
+        bc = g.brandes(None, delta=delta)
+        assert np.sum(bc) / c.n == pytest.approx(fx["brandes_avg"]["value"], abs=1e-6)
+        np.testing.assert_allclose(bc, ref["bc"], rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("seed,n,m", [(1, 60, 400), (2, 300, 1500), (3, 1000, 12000), (4, 2500, 7000)])
+def test_random_integer_graphs_bit_exact(ctx, seed, n, m):
+    c = rand_graph(n, m, seed)
+    g = dev_graph(ctx, c)
+    ref = orc.apsp(c, per_source=True, threads=8)["per_source"]
+    ref_dist = np.stack([orc.sssp(c, s)[0] for s in range(min(n, 64))])
+    for delta in (0.0, 1, 2, 7, 100):
+        s, dist = g.sssp(np.arange(min(n, 64)), delta=delta, want_dist=True)
+        np.testing.assert_array_equal(dist, ref_dist)
+        full = g.sssp(None, delta=delta)
+        np.testing.assert_array_equal(full["sum_d_u64"], ref["sum_d_u64"])
+        np.testing.assert_array_equal(full["max_d_u64"], ref["max_d_u64"])
+        np.testing.assert_array_equal(full["n_unreached"], ref["n_unreached"])
+        np.testing.assert_allclose(full["sum_eff"], ref["sum_eff"], rtol=1e-12)
+
+
+@pytest.mark.parametrize("wdtype", [np.float32, np.float64])
+def test_random_real_graphs_bit_exact(ctx, wdtype):
+    c = rand_graph(1500, 15000, 11, wdtype=wdtype, parallel=False, self_loops=False)
+    g = dev_graph(ctx, c)
+    srcs = np.arange(0, 1500, 7)
+    ref_dist = np.stack([orc.sssp(c, int(s))[0] for s in srcs])
+    ref_sum = orc.apsp(c, sources=srcs, per_source=True)["per_source"]
+    for delta in (0.0, 0.05, 0.3, 1.0, 5.0):
+        s, dist = g.sssp(srcs, delta=delta, want_dist=True)
+        np.testing.assert_array_equal(dist, ref_dist)
+        np.testing.assert_allclose(s["sum_d"], ref_sum["sum_d"], rtol=1e-12)
+        np.testing.assert_array_equal(s["max_d"], ref_sum["max_d"])
+        np.testing.assert_array_equal(s["n_unreached"], ref_sum["n_unreached"])
+
+
+def test_inverse_metric_bit_exact(ctx):
+    from neurograph_amd import NG_LEN_INVERSE
+
+    c = rand_graph(800, 6000, 5, wlo=-4, whi=9)  # zero weights become infinitely long, negatives use |w|
+    g = dev_graph(ctx, c)
+    srcs = np.arange(0, 800, 5)
+    ref = np.stack([orc.sssp(c, int(s), orc.L_INVERSE)[0] for s in srcs])
+    _, dist = g.sssp(srcs, lkind=NG_LEN_INVERSE, want_dist=True)
+    np.testing.assert_array_equal(dist, ref)
+
+
+@pytest.mark.parametrize("wdtype,rtol", [(np.int32, 1e-9), (np.float32, 1e-9), (np.float64, 1e-9)])
+def test_brandes_random_graphs(ctx, wdtype, rtol):
+    c = rand_graph(700, 5000, 21, wdtype=wdtype, wlo=1, whi=4, parallel=True, self_loops=True)
+    g = dev_graph(ctx, c)
+    ref = orc.brandes(c, threads=8, per_source=True)
+    for delta in (0.0, 1, 3):
+        bc, summ = g.brandes(None, delta=delta, summaries=True)
+        np.testing.assert_allclose(bc, ref["bc"], rtol=rtol, atol=1e-9)
+        np.testing.assert_array_equal(summ["n_unreached"], ref["per_source"]["n_unreached"])
+        np.testing.assert_allclose(summ["sum_d"], ref["per_source"]["sum_d"], rtol=1e-12)
+
+
+def test_brandes_is_deterministic_and_accumulates(ctx):
+    c = rand_graph(900, 9000, 8, wdtype=np.float32)
+    g = dev_graph(ctx, c)
+    a = g.brandes(None)
+    b = g.brandes(None)
+    np.testing.assert_array_equal(a, b)
+    half = g.brandes(np.arange(450))
+    both = g.brandes(np.arange(450, 900), bc=half.copy())
+    np.testing.assert_allclose(both, a, rtol=1e-12, atol=1e-12)
+
+
+def test_edge_dependency_and_adjoint_match_oracle(ctx):
+    c = rand_graph(400, 3000, 31, wdtype=np.float64, parallel=False, self_loops=False)
+    g = dev_graph(ctx, c)
+    one = orc.brandes(c, edge=True)
+    inv = orc.brandes(c, edge=True, seed=orc.SEED_INV_D2)
+    edge = g.edge_dependency(None)
+    np.testing.assert_allclose(edge, one["edge"], rtol=1e-9, atol=1e-12)
+    g1, g2 = g.adjoint(None)
+    np.testing.assert_allclose(g1, one["edge"], rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(g2, -inv["edge"], rtol=1e-9, atol=1e-12)
+
+
+def test_integer_ties_edge_dependency(ctx):
+    c = rand_graph(300, 2500, 41, wlo=1, whi=3)  # many ties: sigma > 1
+    g = dev_graph(ctx, c)
+    ref = orc.brandes(c, edge=True)
+    np.testing.assert_allclose(g.edge_dependency(None), ref["edge"], rtol=1e-9, atol=1e-12)
+
+
+def test_error_paths(ctx):
+    import neurograph_amd as ng
+
+    c = rand_graph(50, 200, 2)
+    g = dev_graph(ctx, c)
+    with pytest.raises(ng.NgError) as e:
+        g.sssp([50])
+    assert e.value.status == 1
+    neg = orc.Csr.from_edges(3, [0, 1], [1, 2], [-1, 2])
+    gn = dev_graph(ctx, neg)
+    with pytest.raises(ng.NgError) as e:
+        gn.sssp([0], lkind=ng.NG_LEN_RAW)
+    assert e.value.status == 1
+    assert int(gn.sssp([0])["sum_d_u64"][0]) == 1 + 3  # |w| for the Dijkstra metric
+    zero = orc.Csr.from_edges(3, [0, 1], [1, 2], [0.0, 2.0], dtype=np.float64)
+    gz = dev_graph(ctx, zero)
+    with pytest.raises(ng.NgError) as e:
+        gz.brandes(None)
+    assert e.value.status == 4
+    empty = g.sssp(np.zeros(0, np.uint32))
+    assert len(empty) == 0
+
+
+def test_fixed_degree_config_parity(ctx):
+    """5000 nodes, exactly 6 distinct out-arcs, integer weights in [1, 10], seed 0: every source."""
+    import neurograph_amd as ng
+    from neurograph_amd.metrics import aggregate_apsp
+
+    hg = ng.generate_fixed_degree(5000, 6, 1, 10, 0)
+    c = orc.Csr(hg.n, hg.offsets, hg.targets, hg.weights)
+    g = ng.Graph(ctx, hg)
+    s = g.sssp(None)
+    ref = orc.apsp(c, threads=8, per_source=True)
+    np.testing.assert_array_equal(s["sum_d_u64"], ref["per_source"]["sum_d_u64"])
+    np.testing.assert_array_equal(s["max_d_u64"], ref["per_source"]["max_d_u64"])
+    np.testing.assert_array_equal(s["n_unreached"], ref["per_source"]["n_unreached"])
+    agg = aggregate_apsp(s, hg.n, True)
+    assert agg.average_shortest_path_length == ref["avg_spl"]
+    assert agg.diameter == ref["diameter_u64"]
+    assert agg.number_disconnected_pairs == ref["n_disconnected"]
+    assert agg.cluster_sizes == ref["cluster_sizes"]
+    assert agg.average_efficiency == pytest.approx(ref["avg_eff"], rel=1e-12)
