@@ -63,6 +63,9 @@ typedef struct {
     uint32_t slots;          /* concurrent source slots (workgroups) used */
     double delta;            /* bucket width used */
     uint64_t arcs_reached;   /* sum over sources of out-arcs of reached nodes (TEPS numerator) */
+    uint32_t flags;          /* informational device flags of the last call (2: tight-arc log overflowed
+                                and the full-row scan was used; 4: in-bucket work after a settle pass) */
+    uint32_t reserved;
 } ng_call_stats;
 
 typedef struct ng_context ng_context;

@@ -43,7 +43,8 @@ assert SUMMARY_DTYPE.itemsize == ctypes.sizeof(Summary)
 
 class CallStats(ctypes.Structure):
     _fields_ = [("kernel_ms", ctypes.c_double), ("launches", ctypes.c_uint32), ("slots", ctypes.c_uint32),
-                ("delta", ctypes.c_double), ("arcs_reached", ctypes.c_uint64)]
+                ("delta", ctypes.c_double), ("arcs_reached", ctypes.c_uint64), ("flags", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
 
 
 class HostCsrC(ctypes.Structure):

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -409,6 +410,11 @@ ng_status run_batch(ng_graph* g, const RunSpec& rs, uint32_t* slots_used, double
     // slots: resident workgroups of this kernel, bounded by the sources and by device memory
     const int bpc = batch_blocks_per_cu(v.dk, rs.mode);
     uint64_t slots = static_cast<uint64_t>(bpc) * static_cast<uint64_t>(ctx->cus);
+    // NG_MAX_SLOTS caps the concurrent source slots (for tuning sweeps and experiments)
+    if (const char* cap = std::getenv("NG_MAX_SLOTS")) {
+        const long long c = std::atoll(cap);
+        if (c > 0) slots = std::min<uint64_t>(slots, static_cast<uint64_t>(c));
+    }
     slots = std::min<uint64_t>(slots, std::max<uint32_t>(rs.n_sources, 1));
     size_t free_b = 0, total_b = 0;
     NG_HIP(hipMemGetInfo(&free_b, &total_b));
@@ -475,6 +481,7 @@ ng_status run_batch(ng_graph* g, const RunSpec& rs, uint32_t* slots_used, double
     g->last.slots = static_cast<uint32_t>(slots);
     g->last.delta = delta;
     g->last.arcs_reached = arcs;
+    g->last.flags = err;
     if (slots_used) *slots_used = static_cast<uint32_t>(slots);
     return NG_OK;
 }

@@ -700,6 +700,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
         if (a.dist_out != nullptr) {
             for (uint32_t v = tid; v < n; v += kBlock)
                 write_dist_out(a, static_cast<size_t>(si) * n + v, ldc(&dist[v]));
+            bsync();  // every lane has read its distances before any lane resets them below
         }
         // ---- restore the slot: only reached nodes were touched (after an abort, frontier and pile nodes
         // may hold distances too, so the whole array is reset)
