@@ -1,0 +1,191 @@
+"""Benchmark of the MI355X APSP + Brandes path.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload spatial50k|spatial200k|fixed5k]
+
+One step is one batched call of the hot path (ng_brandes_accumulate with per-source SSSP summaries, so
+every source yields both its APSP summary and its betweenness contribution) over one batch of sources of
+a synthetic graph resident in HBM. The timed region is exactly K steps, bracketed by device
+synchronisation; value = sources processed / wall time.
+
+Workloads (default spatial50k):
+  fixed5k      5000 nodes, exactly 6 distinct out-arcs per node, integer weights U{1..10}, seed 0, all sources
+  spatial50k   50k nodes uniform in a 1000^3 um box, p ~ exp(-d/150 um), mean out-degree 20, FP64 lognormal(0,0.5),
+               all 50k sources
+  spatial200k  200k nodes, mean out-degree 100 (~20M arcs), FP32 lognormal(0,0.5), 8192 sampled sources, batches of 1024
+
+Roofline: algorithmic bytes per source B_bc = 2*(A*(s_i+s_w) + (V+1)*s_o) + V*(2*s_d + 32) (one pass over the
+CSR for the forward sweep, one for the backward sweep, per-node distance/sigma/delta/betweenness state) divided
+by the batched kernel's launch duration (HIP events on the library's stream), against 8 TB/s.
+CPU baseline: the oracle's restatement of the reference (PriorityQueueVariant APSP + BrandesVariant), one
+worker thread per core over contiguous source blocks, timed on a bounded sample of the same sources.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "sources/sec + GTEPS for APSP & Brandes BC, GPU vs host CPU; HBM GB/s fraction"
+HBM_PEAK_GBS = 8000.0
+
+
+def make_workload(name: str):
+    import neurograph_amd as ng
+
+    if name == "fixed5k":
+        g = ng.generate_fixed_degree(5000, 6, 1, 10, 0)
+        return g, np.arange(g.n, dtype=np.uint32), 5000, {"nodes": 5000, "out_degree": 6, "weights": "int U{1..10}"}
+    if name == "spatial50k":
+        g = ng.generate_spatial(50000, 20, box=1000.0, lam=150.0, wdist="lognormal", a=0.0, b=0.5, weights=np.float64, seed=0)
+        return g, np.arange(g.n, dtype=np.uint32), 10000, {"nodes": 50000, "mean_out_degree": 20, "weights": "f64 lognormal(0,0.5)"}
+    if name == "spatial200k":
+        g = ng.generate_spatial(200000, 100, box=1000.0, lam=150.0, wdist="lognormal", a=0.0, b=0.5, weights=np.float32, seed=0)
+        rng = np.random.default_rng(0)
+        src = np.sort(rng.choice(g.n, size=8192, replace=False)).astype(np.uint32)
+        return g, src, 1024, {"nodes": 200000, "mean_out_degree": 100, "weights": "f32 lognormal(0,0.5)", "sampled_sources": 8192}
+    raise SystemExit(f"unknown workload {name}")
+
+
+def bytes_per_source(g) -> float:
+    """B_bc with field sizes s_i = 4 (target), s_w = weight bytes, s_o = 4 (row offset), s_d = distance bytes."""
+    s_w = g.weights.dtype.itemsize
+    s_d = 8 if g.weights.dtype != np.float32 else 4
+    return 2.0 * (g.m * (4 + s_w) + (g.n + 1) * 4) + g.n * (2 * s_d + 32)
+
+
+def cpu_baseline(g, sources, seconds: float):
+    """Oracle restatement of the reference's CPU path (APSP via PriorityQueueVariant + BrandesVariant), all host
+    cores, on a bounded sample of the same sources (sources are independent, so the rate scales linearly)."""
+    from oracle import oracle as orc
+
+    c = orc.Csr(g.n, g.offsets, g.targets, g.weights)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    probe = sources[: max(threads, 1)]
+    t0 = time.perf_counter()
+    orc.apsp(c, sources=probe, threads=threads)
+    orc.brandes(c, sources=probe, threads=threads)
+    per_batch = time.perf_counter() - t0
+    per_source_wall = per_batch / len(probe)
+    k = int(max(len(probe), min(len(sources), seconds / max(per_source_wall, 1e-9))))
+    k = max(threads, (k // threads) * threads)
+    sample = sources[:k]
+    t0 = time.perf_counter()
+    orc.apsp(c, sources=sample, threads=threads)
+    t1 = time.perf_counter()
+    orc.brandes(c, sources=sample, threads=threads)
+    t2 = time.perf_counter()
+    total = t2 - t0
+    return {"value": round(len(sample) / total, 3), "unit": "sources/s", "cores": threads, "kind": "port",
+            "sample": f"{len(sample)} of the workload's sources; avg-apsp {t1 - t0:.2f} s + avg-centrality {t2 - t1:.2f} s "
+                      f"(oracle/oracle.cpp, {threads} threads, contiguous source blocks)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=0, help="timed steps (default: one pass over the workload's sources)")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="spatial50k", choices=["fixed5k", "spatial50k", "spatial200k"])
+    ap.add_argument("--batch", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--delta", type=float, default=0.0, help="bucket width (0: autotune)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (sources would be split across GPUs with "
+              "one final reduction of the betweenness vector; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import neurograph_amd as ng
+
+    t0 = time.perf_counter()
+    hg, sources, batch, cfg = make_workload(args.workload)
+    gen_s = time.perf_counter() - t0
+    batch = args.batch or batch
+    batches = [sources[i:i + batch] for i in range(0, len(sources), batch)]
+    steps = args.steps or len(batches)
+    ctx = ng.Context(0)
+    g = ng.Graph(ctx, hg)
+    print(f"[bench] {args.workload}: n={hg.n} m={hg.m} generated in {gen_s:.1f}s; {len(sources)} sources, batch {batch}, "
+          f"{steps} steps", file=sys.stderr, flush=True)
+
+    delta = args.delta
+    if delta <= 0:
+        rng = np.random.default_rng(1)
+        sample = np.sort(rng.choice(sources, size=min(len(sources), 512), replace=False))
+        delta = g.autotune(sample, mode=1)
+    print(f"[bench] delta {delta:.6g}", file=sys.stderr, flush=True)
+
+    try:
+        import torch
+
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:  # torch is plumbing only
+        sync = lambda: None  # noqa: E731
+
+    bc = np.zeros(hg.n)
+    for w in range(args.warmup):
+        g.brandes(batches[w % len(batches)], delta=delta, bc=bc, summaries=True)
+    bc[:] = 0.0
+    kernel_ms, arcs, srcs = [], 0, 0
+    summaries = []
+    sync()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        b = batches[k % len(batches)]
+        _, summ = g.brandes(b, delta=delta, bc=bc, summaries=True)
+        st = g.last_stats()
+        kernel_ms.append(st["kernel_ms"])
+        arcs += st["arcs_reached"]
+        srcs += len(b)
+        summaries.append(summ)
+    sync()
+    wall = time.perf_counter() - t0
+
+    value = srcs / wall
+    launches_bytes = [bytes_per_source(hg) * len(batches[k % len(batches)]) for k in range(steps)]
+    achieved = sum(launches_bytes) / (sum(kernel_ms) / 1e3) / 1e9
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    if os.path.exists(prof):
+        with open(prof) as f:
+            tr = json.load(f)
+        traffic = tr.get("hbm_bytes_per_launch")
+    res = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "sources/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        # distances are computed in this type; sigma, delta and betweenness always in f64
+        "dtype": {np.dtype(np.int32): "u32", np.dtype(np.float32): "f32", np.dtype(np.float64): "f64"}[hg.weights.dtype],
+        "data": "synthetic",
+        "config": {"workload": args.workload, **cfg, "arcs": hg.m, "sources_per_step": batch, "delta": delta,
+                   "outputs": "per-source APSP summaries + betweenness in one pass"},
+        "gteps": round(arcs / wall / 1e9, 3),
+        "kernel_ms_per_step": round(sum(kernel_ms) / steps, 3),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+    }
+    if not args.no_cpu:
+        print("[bench] timing the CPU baseline", file=sys.stderr, flush=True)
+        res["cpu_baseline"] = cpu_baseline(hg, sources, args.cpu_seconds)
+        res["speedup_vs_cpu"] = round(value / res["cpu_baseline"]["value"], 2)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()
