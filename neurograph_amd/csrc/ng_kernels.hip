@@ -22,8 +22,8 @@
 // BrandesVariant.h:149-154, clearing the bitmap on the way. Nothing is allocated per source: every
 // per-slot array is restored to its clean state by touching only the reached nodes.
 //
-// Reads of data that other lanes modify with atomics go through ldc() (an agent-scope relaxed atomic
-// load, which bypasses the CU's L1); phase boundaries are bsync() (agent fence + workgroup barrier).
+// Reads of data that other lanes modify with atomics go through ldc() (a workgroup-scope relaxed atomic
+// load); phase boundaries are bsync() (a workgroup barrier).
 
 #include "ng_internal.h"
 
@@ -93,18 +93,44 @@ struct DT<double> {
 // ---------------------------------------------------------------------------------------------------
 // memory helpers
 
+// Every per-slot array is touched by exactly one workgroup, so workgroup scope is sufficient for all of
+// them: the atomics are performed in the XCD's L2 and the coherent loads read there, and no phase
+// barrier ever needs an agent-scope fence (which on a multi-XCD device writes back and invalidates L2).
+// Only the outputs shared by all workgroups (gradients, counters, error flags) use agent scope.
 template <class T>
 __device__ __forceinline__ T ldc(const T* p) {
-    return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 template <class T>
 __device__ __forceinline__ void stc(T* p, T v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
-__device__ __forceinline__ void bsync() {
-    __threadfence();
-    __syncthreads();
+template <class T>
+__device__ __forceinline__ T wg_min(T* p, T v) {
+    return __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
+template <class T>
+__device__ __forceinline__ T wg_exch(T* p, T v) {
+    return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <class T>
+__device__ __forceinline__ T wg_add(T* p, T v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <class T>
+__device__ __forceinline__ T wg_sub(T* p, T v) {
+    return __hip_atomic_fetch_sub(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <class T>
+__device__ __forceinline__ T wg_or(T* p, T v) {
+    return __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <class T>
+__device__ __forceinline__ T wg_and(T* p, T v) {
+    return __hip_atomic_fetch_and(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+// Phase boundary: a workgroup barrier with workgroup-scope release/acquire.
+__device__ __forceinline__ void bsync() { __syncthreads(); }
 
 // ---------------------------------------------------------------------------------------------------
 // workgroup primitives (64-wide waves, kBlock lanes)
@@ -367,13 +393,13 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
             const B cur = ldc(&dist[t]);
             bool pf = false, pp = false;
             if (nd < cur) {
-                const B old = atomicMin(&dist[t], nd);
+                const B old = wg_min(&dist[t], nd);
                 if (nd < old) {
                     if (nd < hi) {
-                        pf = atomicExch(&fstamp[t], nph) != nph;
+                        pf = wg_exch(&fstamp[t], nph) != nph;
                     } else {
                         atomicMin(&S.runmin, nd);
-                        pp = atomicExch(&pstamp[t], pgen) != pgen;
+                        pp = wg_exch(&pstamp[t], pgen) != pgen;
                     }
                 }
             }
@@ -418,7 +444,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                     uint32_t v = 0;
                     if (i < nF) {
                         v = Fc[i];
-                        fresh = atomicExch(&sstamp[v], gen) != gen;
+                        fresh = wg_exch(&sstamp[v], gen) != gen;
                     }
                     push(fresh, v, Sl, &S.cnt[2], n);
                 }
@@ -494,10 +520,10 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                     if (ldc(&sstamp[t]) != gen) {
                         const B d = ldc(&dist[t]);
                         if (d < hi) {
-                            tf = atomicExch(&fstamp[t], nph) != nph;
+                            tf = wg_exch(&fstamp[t], nph) != nph;
                         } else {
                             atomicMin(&S.runmin, d);
-                            tp = atomicExch(&pstamp[t], pgen) != pgen;
+                            tp = wg_exch(&pstamp[t], pgen) != pgen;
                         }
                     }
                 }
@@ -588,8 +614,8 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                 const uint32_t t = a.tgt[e];
                 const B c = T::add(ldc(&dist[u]), a.len[e]);
                 if (c == ldc(&dist[t])) {
-                    atomicOr(&bits[e >> 5], 1u << (e & 31));
-                    atomicAdd(&indeg[t], 1u);
+                    wg_or(&bits[e >> 5], 1u << (e & 31));
+                    wg_add(&indeg[t], 1u);
                 }
             };
             if (nlog <= a.logcap) {
@@ -631,8 +657,8 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                                 const uint32_t e = (wi << 5) + static_cast<uint32_t>(__ffs(w) - 1);
                                 w &= w - 1;
                                 const uint32_t t = a.tgt[e];
-                                unsafeAtomicAdd(&sigma[t], su);
-                                if (atomicSub(&indeg[t], 1u) == 1u) {
+                                wg_add(&sigma[t], su);
+                                if (wg_sub(&indeg[t], 1u) == 1u) {
                                     const uint32_t pos = atomicAdd(&S.cnt[4], 1u);
                                     if (pos < n) order[pos] = t;
                                 }
@@ -653,7 +679,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                     stc(&indeg[u], 0u);
                     const uint32_t b = a.off[u], e_end = a.off[u + 1];
                     if (b < e_end)
-                        for (uint32_t wi = b >> 5; wi <= (e_end - 1) >> 5; ++wi) atomicAnd(&bits[wi], ~range_mask(wi, b, e_end));
+                        for (uint32_t wi = b >> 5; wi <= (e_end - 1) >> 5; ++wi) wg_and(&bits[wi], ~range_mask(wi, b, e_end));
                 }
             }
             bsync();
@@ -669,7 +695,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                         for (uint32_t wi = b >> 5; wi <= (e_end - 1) >> 5; ++wi) {
                             uint32_t w = ldc(&bits[wi]) & range_mask(wi, b, e_end);
                             if (w == 0u) continue;
-                            atomicAnd(&bits[wi], ~w);
+                            wg_and(&bits[wi], ~w);
                             while (w) {
                                 const uint32_t e = (wi << 5) + static_cast<uint32_t>(__ffs(w) - 1);
                                 w &= w - 1;
