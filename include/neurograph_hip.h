@@ -66,6 +66,13 @@ typedef struct {
     uint32_t flags;          /* informational device flags of the last call (2: tight-arc log overflowed
                                 and the full-row scan was used; 4: in-bucket work after a settle pass) */
     uint32_t reserved;
+    /* work counters of the last call, summed over sources */
+    uint64_t phases;         /* frontier phases (light-arc rounds) */
+    uint64_t buckets;        /* bucket advances */
+    uint64_t light_arcs;     /* arcs scanned in frontier phases */
+    uint64_t settle_arcs;    /* arcs scanned in settle passes */
+    uint64_t logged_arcs;    /* candidate tight arcs logged (betweenness modes) */
+    uint64_t layers;         /* Kahn layers of the tight DAG (betweenness modes) */
 } ng_call_stats;
 
 typedef struct ng_context ng_context;

@@ -44,7 +44,9 @@ assert SUMMARY_DTYPE.itemsize == ctypes.sizeof(Summary)
 class CallStats(ctypes.Structure):
     _fields_ = [("kernel_ms", ctypes.c_double), ("launches", ctypes.c_uint32), ("slots", ctypes.c_uint32),
                 ("delta", ctypes.c_double), ("arcs_reached", ctypes.c_uint64), ("flags", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32)]
+                ("reserved", ctypes.c_uint32), ("phases", ctypes.c_uint64), ("buckets", ctypes.c_uint64),
+                ("light_arcs", ctypes.c_uint64), ("settle_arcs", ctypes.c_uint64), ("logged_arcs", ctypes.c_uint64),
+                ("layers", ctypes.c_uint64)]
 
 
 class HostCsrC(ctypes.Structure):

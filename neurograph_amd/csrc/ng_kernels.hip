@@ -342,6 +342,8 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
     uint32_t* ctr = a.counters + static_cast<size_t>(slot) * 4;
     uint32_t phase = ctr[0], gen = ctr[1], pgen = ctr[2];
     const L delta = a.delta;
+    // work counters (uniform per workgroup, except the arc counts which lane 0 accumulates from the tiles)
+    unsigned long long c_phases = 0, c_buckets = 0, c_light = 0, c_settle = 0, c_logged = 0, c_layers = 0;
 
     for (uint32_t si = blockIdx.x; si < a.n_sources; si += gridDim.x) {
         if (phase > 0xE0000000u || gen > 0xE0000000u || pgen > 0xE0000000u) {
@@ -429,12 +431,14 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                     break;
                 }
                 const uint32_t nph = phase + 1;
+                ++c_phases;
                 expand(
                     Fc, nF, S,
                     [&](uint32_t v, B& dv, uint32_t& beg, uint32_t& cnt) {
                         dv = ldc(&dist[v]);
                         beg = a.off[v];
                         cnt = a.lend[v] - beg;
+                        c_light += cnt;
                     },
                     [&](uint32_t v, B dv, uint32_t e) { relax(e, dv, nph, false, v); });
                 // first visit in this bucket -> settled list
@@ -470,6 +474,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                         dv = ldc(&dist[v]);
                         beg = (MODE != MODE_SSSP) ? a.off[v] : a.lend[v];
                         cnt = a.off[v + 1] - beg;
+                        c_settle += cnt;
                     },
                     [&](uint32_t v, B dv, uint32_t e) { relax(e, dv, nph, true, v); });
                 bsync();
@@ -496,6 +501,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                 break;
             }
             // ---- next bucket: [max(pile minimum, hi), +delta)
+            ++c_buckets;
             const B rm = S.runmin;
             const B lo = rm > hi ? rm : hi;
             hi = T::sat_add(lo, delta);
@@ -610,6 +616,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
         if constexpr (MODE != MODE_SSSP) if (!aborted) {
             // ---- tight arcs: verify the logged candidates against the final distances
             const uint32_t nlog = S.cnt[3];
+            c_logged += nlog;
             auto mark = [&](uint32_t u, uint32_t e) {
                 const uint32_t t = a.tgt[e];
                 const B c = T::add(ldc(&dist[u]), a.len[e]);
@@ -672,6 +679,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                 bsync();
             }
             if (tid == 0) layers[nl] = le;
+            c_layers += nl;
             if (le != nS) {  // a zero-length tight cycle kept some nodes from being released
                 if (tid == 0) atomicOr(a.err, ERR_KAHN);
                 for (uint32_t i = tid; i < nS; i += kBlock) {
@@ -748,6 +756,17 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
         ctr[0] = phase;
         ctr[1] = gen;
         ctr[2] = pgen;
+        atomicAdd(&a.arcs_reached[1], c_phases);
+        atomicAdd(&a.arcs_reached[2], c_buckets);
+        atomicAdd(&a.arcs_reached[5], c_logged);
+        atomicAdd(&a.arcs_reached[6], c_layers);
+    }
+    // per-lane arc counts (each lane counted the rows it loaded)
+    c_light = wave_reduce(c_light, [](unsigned long long x, unsigned long long y) { return x + y; });
+    c_settle = wave_reduce(c_settle, [](unsigned long long x, unsigned long long y) { return x + y; });
+    if (lane_id() == 0) {
+        atomicAdd(&a.arcs_reached[3], c_light);
+        atomicAdd(&a.arcs_reached[4], c_settle);
     }
 }
 
