@@ -408,7 +408,7 @@ ng_status run_batch(ng_graph* g, const RunSpec& rs, uint32_t* slots_used, double
         v.lend_delta = delta;
     }
     // slots: resident workgroups of this kernel, bounded by the sources and by device memory
-    const int bpc = batch_blocks_per_cu(v.dk, rs.mode);
+    const int bpc = batch_blocks_per_cu(v.dk, rs.mode, g->n);
     uint64_t slots = static_cast<uint64_t>(bpc) * static_cast<uint64_t>(ctx->cus);
     // NG_MAX_SLOTS caps the concurrent source slots (for tuning sweeps and experiments)
     if (const char* cap = std::getenv("NG_MAX_SLOTS")) {

@@ -100,7 +100,7 @@ hipError_t launch_fill_u32(uint32_t* p, uint64_t count, uint32_t value, hipStrea
 hipError_t launch_fill_f64(double* p, uint64_t count, double value, hipStream_t stream);
 hipError_t launch_fill_dist(DKind dk, void* p, uint64_t count, hipStream_t stream);  // fills with +infinity / max
 // Resident workgroups per CU of the batched kernel for (dk, mode).
-int batch_blocks_per_cu(DKind dk, int mode);
+int batch_blocks_per_cu(DKind dk, int mode, uint32_t n);
 constexpr int kBlock = 256;
 
 }  // namespace ng

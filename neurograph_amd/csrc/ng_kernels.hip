@@ -260,6 +260,7 @@ struct KArgs {
     uint64_t words;
     uint2* log;
     uint32_t logcap;
+    uint32_t bm_words;  // words of the LDS bitmap of finalized nodes (0: no bitmap)
     double* bc_slot;
     double* grad1;
     double* grad2;
@@ -304,6 +305,11 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
     using B = typename T::B;
     using L = typename T::L;
     __shared__ Lds<B> S;
+    // Bitmap (dynamic LDS) of the nodes finalized in earlier buckets of the current source: an arc into
+    // such a node can neither improve it nor be tight (its distance is below the bucket, the candidate is
+    // not), so its global distance probe is skipped.
+    extern __shared__ uint32_t bm[];
+    const bool use_bm = a.bm_words != 0;
 
     const uint32_t tid = threadIdx.x;
     const uint32_t slot = blockIdx.x;
@@ -358,6 +364,8 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
         ++gen;
         ++pgen;
         ++phase;
+        if (use_bm)
+            for (uint32_t w = tid; w < a.bm_words; w += kBlock) bm[w] = 0u;
         const uint32_t s = a.sources[si];
         if (tid == 0) {
 #pragma unroll
@@ -372,6 +380,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
         uint32_t* Pn = Pb;
         uint32_t nF = 1;
         uint32_t settle_from = 0;
+        uint32_t bucket_begin = 0;
         B hi = T::sat_add(B(0), delta);
         // Uniform iteration guard: phases and bucket advances are bounded, so a runaway loop (a bug, or a
         // pathological input) ends with ERR_GUARD instead of occupying the device.
@@ -392,7 +401,8 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
         auto relax = [&](uint32_t e, B dv, uint32_t nph, bool log_it_if_close, uint32_t v) {
             const uint32_t t = a.tgt[e];
             const B nd = T::add(dv, a.len[e]);
-            const B cur = ldc(&dist[t]);
+            const bool final_t = use_bm && ((bm[t >> 5] >> (t & 31)) & 1u);
+            const B cur = final_t ? B(0) : ldc(&dist[t]);
             bool pf = false, pp = false;
             if (nd < cur) {
                 const B old = wg_min(&dist[t], nd);
@@ -408,7 +418,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
             push(pf, t, Fn, &S.cnt[0], n);
             push(pp, t, Pc, &S.cnt[1], n);
             if constexpr (MODE != MODE_SSSP) {
-                const bool lg = log_it_if_close && nd <= cur;
+                const bool lg = log_it_if_close && !final_t && nd <= cur;
                 const unsigned long long mask = __ballot(lg);
                 if (mask != 0ull) {
                     const uint32_t lane = lane_id();
@@ -493,6 +503,14 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                     continue;
                 }
             }
+            // the bucket is complete: its nodes join the finalized bitmap
+            if (use_bm) {
+                for (uint32_t i = bucket_begin + tid; i < settle_from; i += kBlock) {
+                    const uint32_t v = Sl[i];
+                    atomicOr(&bm[v >> 5], 1u << (v & 31));
+                }
+            }
+            bucket_begin = settle_from;
             const uint32_t nP = clampn(S.cnt[1]);
             if (aborted || nP == 0) break;
             if (++guard > guard_max) {
@@ -807,6 +825,12 @@ hipError_t fill(T* p, uint64_t count, T value, hipStream_t stream) {
     return hipGetLastError();
 }
 
+// Words of the finalized-node bitmap kept in LDS, or 0 when it would not fit next to the static LDS.
+uint32_t bitmap_words(uint32_t n) {
+    const uint32_t w = (n + 31) / 32;
+    return w * 4 <= 96 * 1024 ? w : 0u;
+}
+
 template <class D, int MODE>
 hipError_t launch_typed(const LaunchArgs& la, hipStream_t stream) {
     using T = DT<D>;
@@ -853,7 +877,8 @@ hipError_t launch_typed(const LaunchArgs& la, hipStream_t stream) {
     k.err = ws.err;
     k.dist_out = la.dist_out;
     k.dist_out_kind = la.dist_out_kind;
-    hipLaunchKernelGGL((batch_kernel<D, MODE>), dim3(la.grid), dim3(kBlock), 0, stream, k);
+    k.bm_words = bitmap_words(la.n);
+    hipLaunchKernelGGL((batch_kernel<D, MODE>), dim3(la.grid), dim3(kBlock), k.bm_words * 4, stream, k);
     return hipGetLastError();
 }
 
@@ -869,15 +894,16 @@ hipError_t launch_mode(const LaunchArgs& a, hipStream_t stream) {
 }
 
 template <class D>
-int blocks_per_cu_mode(int mode) {
+int blocks_per_cu_mode(int mode, uint32_t n) {
     int nb = 0;
     hipError_t e = hipErrorInvalidValue;
+    const size_t dyn = static_cast<size_t>(bitmap_words(n)) * 4;
     switch (mode) {
-    case MODE_SSSP: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<D, MODE_SSSP>, kBlock, 0); break;
-    case MODE_BC: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<D, MODE_BC>, kBlock, 0); break;
-    case MODE_ARC1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<D, MODE_ARC1>, kBlock, 0); break;
+    case MODE_SSSP: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<D, MODE_SSSP>, kBlock, dyn); break;
+    case MODE_BC: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<D, MODE_BC>, kBlock, dyn); break;
+    case MODE_ARC1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<D, MODE_ARC1>, kBlock, dyn); break;
     default:
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<D, MODE_ARC1 | MODE_ARC2>, kBlock, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<D, MODE_ARC1 | MODE_ARC2>, kBlock, dyn);
         break;
     }
     return (e == hipSuccess && nb > 0) ? nb : 1;
@@ -895,12 +921,12 @@ hipError_t launch_batch(const LaunchArgs& a, hipStream_t stream) {
     return hipErrorInvalidValue;
 }
 
-int batch_blocks_per_cu(DKind dk, int mode) {
+int batch_blocks_per_cu(DKind dk, int mode, uint32_t n) {
     switch (dk) {
-    case DKind::U32: return blocks_per_cu_mode<uint32_t>(mode);
-    case DKind::U64: return blocks_per_cu_mode<uint64_t>(mode);
-    case DKind::F32: return blocks_per_cu_mode<float>(mode);
-    case DKind::F64: return blocks_per_cu_mode<double>(mode);
+    case DKind::U32: return blocks_per_cu_mode<uint32_t>(mode, n);
+    case DKind::U64: return blocks_per_cu_mode<uint64_t>(mode, n);
+    case DKind::F32: return blocks_per_cu_mode<float>(mode, n);
+    case DKind::F64: return blocks_per_cu_mode<double>(mode, n);
     }
     return 1;
 }
