@@ -231,6 +231,80 @@ __device__ __forceinline__ void expand(const uint32_t* list, uint32_t count, Lds
     }
 }
 
+// Arc loop of the relaxation passes with U arcs in flight per lane: owners are located for all U arcs,
+// then all U CSR loads are issued, then all U distance probes, and only then are the results applied, so
+// each lane keeps U independent gathers outstanding instead of one. The owner search carries its lower
+// bound because a lane's arc index only grows. apply() runs on every active lane (it may append to
+// queues with wave ballots), with ok = false for padding slots.
+#ifndef NG_UNROLL
+#define NG_UNROLL 4
+#endif
+template <int U, class B, class L, class NodeFn, class ProbeFn, class ApplyFn>
+__device__ __forceinline__ void expand_relax(const uint32_t* list, uint32_t count, Lds<B>& S, const uint32_t* tgt,
+                                             const L* len, NodeFn node_fn, ProbeFn probe, ApplyFn apply) {
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t base = 0; base < count; base += kBlock) {
+        const uint32_t i = base + tid;
+        uint32_t v = 0, beg = 0, cnt = 0;
+        B dv = 0;
+        if (i < count) {
+            v = list[i];
+            node_fn(v, dv, beg, cnt);
+        }
+        uint32_t total;
+        const uint32_t ex = block_excl_scan(cnt, &total, S.wave);
+        S.t_v[tid] = v;
+        S.t_dv[tid] = dv;
+        S.t_beg[tid] = beg;
+        S.t_off[tid] = ex;
+        __syncthreads();
+        uint32_t lo_carry = 0;
+        for (uint32_t k0 = tid; k0 < total; k0 += U * kBlock) {
+            uint32_t pv[U], pe[U], pt[U];
+            B pdv[U], pcur[U];
+            L pl[U];
+            bool pok[U], pfin[U];
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                const uint32_t k = k0 + static_cast<uint32_t>(j) * kBlock;
+                pok[j] = k < total;
+                pv[j] = 0;
+                pe[j] = 0;
+                pdv[j] = 0;
+                pt[j] = 0;
+                pl[j] = L(0);
+                pfin[j] = true;
+                pcur[j] = 0;
+                if (pok[j]) {
+                    uint32_t lo = lo_carry, hi = kBlock;
+                    while (hi - lo > 1) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (S.t_off[mid] <= k) lo = mid; else hi = mid;
+                    }
+                    lo_carry = lo;
+                    pv[j] = S.t_v[lo];
+                    pdv[j] = S.t_dv[lo];
+                    pe[j] = S.t_beg[lo] + (k - S.t_off[lo]);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                if (pok[j]) {
+                    pt[j] = tgt[pe[j]];
+                    pl[j] = len[pe[j]];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                if (pok[j]) probe(pt[j], pfin[j], pcur[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < U; ++j) apply(pok[j], pv[j], pdv[j], pe[j], pt[j], pl[j], pfin[j], pcur[j]);
+        }
+        __syncthreads();
+    }
+}
+
 template <class D>
 struct KArgs {
     using B = typename DT<D>::B;
@@ -397,14 +471,18 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
         };
         bsync();
 
-        // relaxation of arc e from a node at distance dv; new frontier entries are stamped `nph`
-        auto relax = [&](uint32_t e, B dv, uint32_t nph, bool log_it_if_close, uint32_t v) {
-            const uint32_t t = a.tgt[e];
-            const B nd = T::add(dv, a.len[e]);
-            const bool final_t = use_bm && ((bm[t >> 5] >> (t & 31)) & 1u);
-            const B cur = final_t ? B(0) : ldc(&dist[t]);
+        // distance probe of arc target t: skipped (fin) for nodes finalized in earlier buckets
+        auto probe = [&](uint32_t t, bool& fin, B& cur) {
+            fin = use_bm && ((bm[t >> 5] >> (t & 31)) & 1u);
+            cur = fin ? B(0) : ldc(&dist[t]);
+        };
+        // relaxation of arc e = (v, t) with length l from v at distance dv, given the probe (fin, cur); new
+        // frontier entries are stamped `nph`; in the settle pass the candidate tight arcs are logged
+        auto apply = [&](bool ok, uint32_t v, B dv, uint32_t e, uint32_t t, L l, bool final_t, B cur, uint32_t nph,
+                         bool log_it_if_close) {
+            const B nd = T::add(dv, l);
             bool pf = false, pp = false;
-            if (nd < cur) {
+            if (ok && !final_t && nd < cur) {
                 const B old = wg_min(&dist[t], nd);
                 if (nd < old) {
                     if (nd < hi) {
@@ -418,7 +496,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
             push(pf, t, Fn, &S.cnt[0], n);
             push(pp, t, Pc, &S.cnt[1], n);
             if constexpr (MODE != MODE_SSSP) {
-                const bool lg = log_it_if_close && !final_t && nd <= cur;
+                const bool lg = ok && log_it_if_close && !final_t && nd <= cur;
                 const unsigned long long mask = __ballot(lg);
                 if (mask != 0ull) {
                     const uint32_t lane = lane_id();
@@ -442,15 +520,18 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                 }
                 const uint32_t nph = phase + 1;
                 ++c_phases;
-                expand(
-                    Fc, nF, S,
+                expand_relax<NG_UNROLL>(
+                    Fc, nF, S, a.tgt, a.len,
                     [&](uint32_t v, B& dv, uint32_t& beg, uint32_t& cnt) {
                         dv = ldc(&dist[v]);
                         beg = a.off[v];
                         cnt = a.lend[v] - beg;
                         c_light += cnt;
                     },
-                    [&](uint32_t v, B dv, uint32_t e) { relax(e, dv, nph, false, v); });
+                    probe,
+                    [&](bool ok, uint32_t v, B dv, uint32_t e, uint32_t t, L l, bool fin, B cur) {
+                        apply(ok, v, dv, e, t, l, fin, cur, nph, false);
+                    });
                 // first visit in this bucket -> settled list
                 for (uint32_t base = 0; base < nF; base += kBlock) {
                     const uint32_t i = base + tid;
@@ -478,15 +559,18 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
             if (aborted) break;
             if (nS > settle_from) {
                 const uint32_t nph = phase + 1;
-                expand(
-                    Sl + settle_from, nS - settle_from, S,
+                expand_relax<NG_UNROLL>(
+                    Sl + settle_from, nS - settle_from, S, a.tgt, a.len,
                     [&](uint32_t v, B& dv, uint32_t& beg, uint32_t& cnt) {
                         dv = ldc(&dist[v]);
                         beg = (MODE != MODE_SSSP) ? a.off[v] : a.lend[v];
                         cnt = a.off[v + 1] - beg;
                         c_settle += cnt;
                     },
-                    [&](uint32_t v, B dv, uint32_t e) { relax(e, dv, nph, true, v); });
+                    probe,
+                    [&](bool ok, uint32_t v, B dv, uint32_t e, uint32_t t, L l, bool fin, B cur) {
+                        apply(ok, v, dv, e, t, l, fin, cur, nph, true);
+                    });
                 bsync();
                 settle_from = nS;
                 nF = clampn(S.cnt[0]);
