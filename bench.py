@@ -98,6 +98,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--delta", type=float, default=0.0, help="bucket width (0: autotune)")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="also run one device copy of this many MiB (a known-size kernel for scaling profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (sources would be split across GPUs with "
@@ -130,6 +132,18 @@ def main():
         sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
     except Exception:  # torch is plumbing only
         sync = lambda: None  # noqa: E731
+
+    if args.copy_calibration_mib:
+        import torch
+
+        x = torch.ones(args.copy_calibration_mib * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(f"[bench] calibration copy: {args.copy_calibration_mib} MiB read + {args.copy_calibration_mib} MiB written",
+              file=sys.stderr, flush=True)
+        del x, y
 
     bc = np.zeros(hg.n)
     for w in range(args.warmup):
