@@ -126,24 +126,7 @@ def main():
         delta = g.autotune(sample, mode=1)
     print(f"[bench] delta {delta:.6g}", file=sys.stderr, flush=True)
 
-    try:
-        import torch
-
-        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:  # torch is plumbing only
-        sync = lambda: None  # noqa: E731
-
-    if args.copy_calibration_mib:
-        import torch
-
-        x = torch.ones(args.copy_calibration_mib * (1 << 20) // 4, dtype=torch.float32, device="cuda")
-        y = torch.empty_like(x)
-        torch.cuda.synchronize()
-        y.copy_(x)
-        torch.cuda.synchronize()
-        print(f"[bench] calibration copy: {args.copy_calibration_mib} MiB read + {args.copy_calibration_mib} MiB written",
-              file=sys.stderr, flush=True)
-        del x, y
+    sync = ctx.synchronize  # every ng call synchronizes already; this brackets the timed region explicitly
 
     bc = np.zeros(hg.n)
     for w in range(args.warmup):

@@ -84,6 +84,14 @@ typedef struct ng_cohort ng_cohort;
 ng_status ng_context_create(int device, ng_context** out);
 void ng_context_destroy(ng_context* ctx);
 
+/* Waits until all work of the context's stream is done (the barrier around timed regions). */
+ng_status ng_context_synchronize(ng_context* ctx);
+
+/* Measurement aid, no reference counterpart: one plain device copy of `bytes` bytes (16-byte vector loads
+ * and stores) on the context stream, a kernel of known traffic against which profiler byte counters are
+ * scaled. ms_out (optional) receives its duration from HIP events. */
+ng_status ng_copy_calibration(ng_context* ctx, uint64_t bytes, double* ms_out);
+
 /* Uploads a graph given as host CSR over out-arcs: out_offsets[n+1] (non-decreasing, out_offsets[0]=0,
  * out_offsets[n]=m), out_targets[m] < n, out_weights[m] of `wkind`. Replaces the per-rank RMA windows
  * and remote adjacency cache: DistributedGraph::construct_graph (DistributedGraph.cpp:46-127),

@@ -71,6 +71,8 @@ def lib() -> ctypes.CDLL:
     sig = {
         "ng_context_create": (i32, [i32, P(vp)]),
         "ng_context_destroy": (None, [vp]),
+        "ng_context_synchronize": (i32, [vp]),
+        "ng_copy_calibration": (i32, [vp, u64, P(dbl)]),
         "ng_graph_create": (i32, [vp, u32, u64, vp, vp, vp, i32, P(vp)]),
         "ng_graph_destroy": (None, [vp]),
         "ng_graph_dist_kind": (i32, [vp, i32, P(i32)]),
@@ -207,6 +209,15 @@ class Context:
         if self.h:
             lib().ng_context_destroy(self.h)
             self.h = None
+
+    def synchronize(self):
+        _check(lib().ng_context_synchronize(self.h))
+
+    def copy_calibration(self, nbytes: int) -> float:
+        """Runs one plain device copy of nbytes; returns its duration in ms."""
+        ms = ctypes.c_double()
+        _check(lib().ng_copy_calibration(self.h, nbytes, ctypes.byref(ms)))
+        return ms.value
 
     def __del__(self):
         try:

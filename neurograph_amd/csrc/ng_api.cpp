@@ -547,6 +547,41 @@ ng_status ng_context_create(int device, ng_context** out) {
     return NG_OK;
 }
 
+ng_status ng_context_synchronize(ng_context* ctx) {
+    if (!ctx) return fail(NG_EINVAL, "null context");
+    NG_HIP(hipSetDevice(ctx->device));
+    NG_HIP(hipStreamSynchronize(ctx->stream));
+    return NG_OK;
+}
+
+ng_status ng_copy_calibration(ng_context* ctx, uint64_t bytes, double* ms_out) {
+    if (!ctx) return fail(NG_EINVAL, "null context");
+    bytes &= ~static_cast<uint64_t>(15);
+    if (bytes == 0) return fail(NG_EINVAL, "copy size must be at least 16 bytes");
+    NG_HIP(hipSetDevice(ctx->device));
+    void* src = nullptr;
+    void* dst = nullptr;
+    NG_HIP(hipMalloc(&src, bytes));
+    hipError_t e = hipMalloc(&dst, bytes);
+    if (e != hipSuccess) {
+        (void)hipFree(src);
+        return fail(NG_ENOMEM, "hipMalloc failed: %s", hipGetErrorString(e));
+    }
+    float ms = 0.0f;
+    e = launch_fill_u32(static_cast<uint32_t*>(src), bytes / 4, 1u, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev0, ctx->stream);
+    if (e == hipSuccess) e = launch_copy(src, dst, bytes, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    if (e != hipSuccess) return fail(NG_EDEVICE, "copy calibration failed: %s", hipGetErrorString(e));
+    if (ms_out) *ms_out = ms;
+    return NG_OK;
+}
+
 void ng_context_destroy(ng_context* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);

@@ -99,6 +99,7 @@ hipError_t launch_reduce_slots(const double* bc_slot, uint32_t slots, uint32_t n
 hipError_t launch_fill_u32(uint32_t* p, uint64_t count, uint32_t value, hipStream_t stream);
 hipError_t launch_fill_f64(double* p, uint64_t count, double value, hipStream_t stream);
 hipError_t launch_fill_dist(DKind dk, void* p, uint64_t count, hipStream_t stream);  // fills with +infinity / max
+hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t stream);  // bytes % 16 == 0
 // Resident workgroups per CU of the batched kernel for (dk, mode).
 int batch_blocks_per_cu(DKind dk, int mode, uint32_t n);
 constexpr int kBlock = 256;

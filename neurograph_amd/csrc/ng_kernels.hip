@@ -1049,6 +1049,19 @@ hipError_t launch_reduce_slots(const double* bc_slot, uint32_t slots, uint32_t n
     return hipGetLastError();
 }
 
+__global__ void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t count) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t stream) {
+    const uint64_t count = bytes / 16;
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(copy_kernel, dim3(8192), dim3(256), 0, stream, static_cast<const uint4*>(src), static_cast<uint4*>(dst),
+                       count);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_u32(uint32_t* p, uint64_t count, uint32_t value, hipStream_t stream) {
     return fill<uint32_t>(p, count, value, stream);
 }
