@@ -73,6 +73,8 @@ typedef struct {
     uint64_t settle_arcs;    /* arcs scanned in settle passes */
     uint64_t logged_arcs;    /* candidate tight arcs logged (betweenness modes) */
     uint64_t layers;         /* Kahn layers of the tight DAG (betweenness modes) */
+    uint64_t probes;         /* arc-target distance probes issued (arcs minus skipped ones) */
+    uint64_t improvements;   /* successful distance improvements */
 } ng_call_stats;
 
 typedef struct ng_context ng_context;

@@ -46,7 +46,7 @@ class CallStats(ctypes.Structure):
                 ("delta", ctypes.c_double), ("arcs_reached", ctypes.c_uint64), ("flags", ctypes.c_uint32),
                 ("reserved", ctypes.c_uint32), ("phases", ctypes.c_uint64), ("buckets", ctypes.c_uint64),
                 ("light_arcs", ctypes.c_uint64), ("settle_arcs", ctypes.c_uint64), ("logged_arcs", ctypes.c_uint64),
-                ("layers", ctypes.c_uint64)]
+                ("layers", ctypes.c_uint64), ("probes", ctypes.c_uint64), ("improvements", ctypes.c_uint64)]
 
 
 class HostCsrC(ctypes.Structure):

@@ -59,7 +59,8 @@ struct Workspace {
     double* bc_slot = nullptr;  // [slots][n]
     void* summaries = nullptr;  // [max_sources] ng_sssp_summary
     uint32_t summaries_cap = 0;
-    uint64_t* arcs_reached = nullptr;  // [8]: arcs reached, phases, buckets, light arcs, settle arcs, logged, layers
+    uint64_t* arcs_reached = nullptr;  // [16]: arcs reached, phases, buckets, light arcs, settle arcs, logged, layers,
+                                       // probes, improvements
     uint32_t* err = nullptr;    // [1]
     uint32_t* sources = nullptr;
     uint32_t sources_cap = 0;

@@ -296,7 +296,7 @@ __device__ __forceinline__ void expand_relax(const uint32_t* list, uint32_t coun
             }
 #pragma unroll
             for (int j = 0; j < U; ++j) {
-                if (pok[j]) probe(pt[j], pfin[j], pcur[j]);
+                if (pok[j]) probe(pt[j], pl[j], pfin[j], pcur[j]);
             }
 #pragma unroll
             for (int j = 0; j < U; ++j) apply(pok[j], pv[j], pdv[j], pe[j], pt[j], pl[j], pfin[j], pcur[j]);
@@ -379,9 +379,10 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
     using B = typename T::B;
     using L = typename T::L;
     __shared__ Lds<B> S;
-    // Bitmap (dynamic LDS) of the nodes finalized in earlier buckets of the current source: an arc into
-    // such a node can neither improve it nor be tight (its distance is below the bucket, the candidate is
-    // not), so its global distance probe is skipped.
+    // Bitmap (dynamic LDS) of the nodes registered in the settled list of the current source (this bucket or
+    // an earlier one). It replaces the global settled stamps, and in a settle pass a heavy arc (length >=
+    // delta) into a registered node skips its global distance probe: the target's distance is final and
+    // below hi while the candidate is at least hi, so the arc can neither improve nor be tight.
     extern __shared__ uint32_t bm[];
     const bool use_bm = a.bm_words != 0;
 
@@ -424,6 +425,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
     const L delta = a.delta;
     // work counters (uniform per workgroup, except the arc counts which lane 0 accumulates from the tiles)
     unsigned long long c_phases = 0, c_buckets = 0, c_light = 0, c_settle = 0, c_logged = 0, c_layers = 0;
+    unsigned long long c_probe = 0, c_improve = 0;
 
     for (uint32_t si = blockIdx.x; si < a.n_sources; si += gridDim.x) {
         if (phase > 0xE0000000u || gen > 0xE0000000u || pgen > 0xE0000000u) {
@@ -454,8 +456,9 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
         uint32_t* Pn = Pb;
         uint32_t nF = 1;
         uint32_t settle_from = 0;
-        uint32_t bucket_begin = 0;
         B hi = T::sat_add(B(0), delta);
+        // heavy arcs land at or beyond hi only while lo + delta does not round back to lo
+        bool skip_ok = true;
         // Uniform iteration guard: phases and bucket advances are bounded, so a runaway loop (a bug, or a
         // pathological input) ends with ERR_GUARD instead of occupying the device.
         uint32_t guard = 0;
@@ -472,9 +475,10 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
         bsync();
 
         // distance probe of arc target t: skipped (fin) for nodes finalized in earlier buckets
-        auto probe = [&](uint32_t t, bool& fin, B& cur) {
-            fin = use_bm && ((bm[t >> 5] >> (t & 31)) & 1u);
+        auto probe = [&](uint32_t t, L l, bool& fin, B& cur) {
+            fin = use_bm && skip_ok && !(l < delta) && ((bm[t >> 5] >> (t & 31)) & 1u);
             cur = fin ? B(0) : ldc(&dist[t]);
+            c_probe += fin ? 0u : 1u;
         };
         // relaxation of arc e = (v, t) with length l from v at distance dv, given the probe (fin, cur); new
         // frontier entries are stamped `nph`; in the settle pass the candidate tight arcs are logged
@@ -485,6 +489,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
             if (ok && !final_t && nd < cur) {
                 const B old = wg_min(&dist[t], nd);
                 if (nd < old) {
+                    ++c_improve;
                     if (nd < hi) {
                         pf = wg_exch(&fstamp[t], nph) != nph;
                     } else {
@@ -539,7 +544,12 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                     uint32_t v = 0;
                     if (i < nF) {
                         v = Fc[i];
-                        fresh = wg_exch(&sstamp[v], gen) != gen;
+                        if (use_bm) {
+                            const uint32_t bit = 1u << (v & 31);
+                            fresh = (atomicOr(&bm[v >> 5], bit) & bit) == 0u;
+                        } else {
+                            fresh = wg_exch(&sstamp[v], gen) != gen;
+                        }
                     }
                     push(fresh, v, Sl, &S.cnt[2], n);
                 }
@@ -587,14 +597,6 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                     continue;
                 }
             }
-            // the bucket is complete: its nodes join the finalized bitmap
-            if (use_bm) {
-                for (uint32_t i = bucket_begin + tid; i < settle_from; i += kBlock) {
-                    const uint32_t v = Sl[i];
-                    atomicOr(&bm[v >> 5], 1u << (v & 31));
-                }
-            }
-            bucket_begin = settle_from;
             const uint32_t nP = clampn(S.cnt[1]);
             if (aborted || nP == 0) break;
             if (++guard > guard_max) {
@@ -607,6 +609,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
             const B rm = S.runmin;
             const B lo = rm > hi ? rm : hi;
             hi = T::sat_add(lo, delta);
+            skip_ok = hi > lo;
             // lo + delta may round back to lo for real-valued distances: the window must not be empty,
             // or the pile would never drain. The next representable value is bits + 1.
             if (hi <= lo && lo != T::kInf) hi = lo + 1;
@@ -625,7 +628,8 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                 uint32_t t = 0;
                 if (i < nP) {
                     t = Pc[i];
-                    if (ldc(&sstamp[t]) != gen) {
+                    const bool settled = use_bm ? ((bm[t >> 5] >> (t & 31)) & 1u) != 0u : ldc(&sstamp[t]) == gen;
+                    if (!settled) {
                         const B d = ldc(&dist[t]);
                         if (d < hi) {
                             tf = wg_exch(&fstamp[t], nph) != nph;
@@ -864,11 +868,16 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
         atomicAdd(&a.arcs_reached[6], c_layers);
     }
     // per-lane arc counts (each lane counted the rows it loaded)
-    c_light = wave_reduce(c_light, [](unsigned long long x, unsigned long long y) { return x + y; });
-    c_settle = wave_reduce(c_settle, [](unsigned long long x, unsigned long long y) { return x + y; });
+    auto add64 = [](unsigned long long x, unsigned long long y) { return x + y; };
+    c_light = wave_reduce(c_light, add64);
+    c_settle = wave_reduce(c_settle, add64);
+    c_probe = wave_reduce(c_probe, add64);
+    c_improve = wave_reduce(c_improve, add64);
     if (lane_id() == 0) {
         atomicAdd(&a.arcs_reached[3], c_light);
         atomicAdd(&a.arcs_reached[4], c_settle);
+        atomicAdd(&a.arcs_reached[7], c_probe);
+        atomicAdd(&a.arcs_reached[8], c_improve);
     }
 }
 
