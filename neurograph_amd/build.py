@@ -42,8 +42,6 @@ def _run(cmd):
 def _obj(src):
     out = os.path.join(BUILD, os.path.basename(src) + ".o")
     cmd = [HIPCC] + COMMON + DEVICE + ["-c", os.path.join(CSRC, src), "-o", out]
-    if src.endswith(".cpp"):
-        cmd = [HIPCC] + COMMON + ["-c", os.path.join(CSRC, src), "-o", out]
     _run(cmd)
     return out
 

@@ -263,6 +263,8 @@ void free_ws(Workspace& w) {
     dfree(w.sstamp);
     dfree(w.pstamp);
     dfree(w.lists);
+    if (w.pile) (void)hipFree(w.pile);
+    w.pile = nullptr;
     dfree(w.counters);
     dfree(w.sigma);
     dfree(w.delta1);
@@ -284,9 +286,12 @@ void free_ws(Workspace& w) {
     w = Workspace{};
 }
 
+// Pile capacity per list: n first parkings plus 2n re-parkings with improved values.
+uint32_t pile_capacity(uint32_t n) { return static_cast<uint32_t>(std::min<uint64_t>(3ull * n + 1024, 0xFFFFFF00u)); }
+
 // Per-slot device bytes of the batched kernel for (n, m, mode).
 uint64_t slot_bytes(uint32_t n, uint64_t m, int mode, size_t dbytes, uint32_t logcap) {
-    uint64_t b = static_cast<uint64_t>(n) * (dbytes + 12 + 20) + 16;
+    uint64_t b = static_cast<uint64_t>(n) * (dbytes + 12 + 12) + 16 + 32ull * pile_capacity(n);
     if (mode != MODE_SSSP) {
         b += static_cast<uint64_t>(n) * (8 + 8 + 4 + 4 + 4 + 8 + 8) + 4 + ((m + 31) / 32) * 4 + static_cast<uint64_t>(logcap) * 8;
     }
@@ -318,7 +323,14 @@ ng_status ensure_ws(ng_graph* g, uint32_t slots, int mode, DKind dk, uint32_t n_
         NG_HIP(hipMalloc(&w.fstamp, 4 * std::max<size_t>(ns, 1)));
         NG_HIP(hipMalloc(&w.sstamp, 4 * std::max<size_t>(ns, 1)));
         NG_HIP(hipMalloc(&w.pstamp, 4 * std::max<size_t>(ns, 1)));
-        NG_HIP(hipMalloc(&w.lists, 20 * std::max<size_t>(ns, 1)));
+        NG_HIP(hipMalloc(&w.lists, 12 * std::max<size_t>(ns, 1)));
+        w.pcap = pile_capacity(n);
+        // NG_PILE_CAPACITY shrinks the pile (tests use it to force the exact bucket-advance path)
+        if (const char* pc = std::getenv("NG_PILE_CAPACITY")) {
+            const long long c = std::atoll(pc);
+            if (c > 0) w.pcap = static_cast<uint32_t>(std::max<long long>(c, static_cast<long long>(n) + 2 * kBlock));
+        }
+        NG_HIP(hipMalloc(&w.pile, 16ull * 2 * w.pcap * slots));
         NG_HIP(hipMalloc(&w.counters, 16 * static_cast<size_t>(slots)));
         NG_HIP(launch_fill_u32(w.fstamp, ns, 0u, st));
         NG_HIP(launch_fill_u32(w.sstamp, ns, 0u, st));

@@ -45,7 +45,9 @@ struct Workspace {
     uint32_t* fstamp = nullptr; // [slots][n]
     uint32_t* sstamp = nullptr; // [slots][n]
     uint32_t* pstamp = nullptr; // [slots][n]
-    uint32_t* lists = nullptr;  // [slots][5][n]: frontier A/B, pile A/B, settled
+    uint32_t* lists = nullptr;  // [slots][3][n]: frontier A/B, settled
+    void* pile = nullptr;       // [slots][2][pcap] pile entries (16 bytes reserved per entry)
+    uint32_t pcap = 0;
     uint32_t* counters = nullptr;  // [slots][4] persistent stamp generations
     double* sigma = nullptr;    // [slots][n]
     double* delta1 = nullptr;   // [slots][n]

@@ -168,6 +168,35 @@ __device__ __forceinline__ void push(bool pred, uint32_t val, uint32_t* list, ui
     if (pred && base + rank < cap) list[base + rank] = val;
 }
 
+// A pile entry: a node parked beyond the current bucket with the distance it had when it was parked.
+// Every later improvement that stays beyond the bucket parks it again, so the newest entry of a node
+// always carries its current distance and a bucket advance needs no distance loads.
+template <class B>
+struct PileEntry {
+    B val;
+    uint32_t node;
+};
+
+// Wave-aggregated append of a pile entry (see push()).
+template <class B>
+__device__ __forceinline__ void push_entry(bool pred, B val, uint32_t node, PileEntry<B>* list, uint32_t* lds_cnt,
+                                           uint32_t cap) {
+    const unsigned long long mask = __ballot(pred);
+    if (mask == 0ull) return;
+    const uint32_t lane = lane_id();
+    const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<long long>(mask)) - 1);
+    const uint32_t rank = static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(lds_cnt, static_cast<uint32_t>(__popcll(mask)));
+    base = __shfl(base, static_cast<int>(leader), 64);
+    if (pred && base + rank < cap) {
+        PileEntry<B> e;
+        e.val = val;
+        e.node = node;
+        list[base + rank] = e;
+    }
+}
+
 template <class B>
 struct Lds {
     uint32_t t_v[kBlock];
@@ -323,6 +352,8 @@ struct KArgs {
     uint32_t* sstamp;
     uint32_t* pstamp;
     uint32_t* lists;
+    void* pile;
+    uint32_t pcap;
     uint32_t* counters;
     double* sigma;
     double* delta1;
@@ -394,12 +425,13 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
     uint32_t* fstamp = a.fstamp + so;
     uint32_t* sstamp = a.sstamp + so;
     uint32_t* pstamp = a.pstamp + so;
-    uint32_t* lists = a.lists + so * 5;
+    uint32_t* lists = a.lists + so * 3;
     uint32_t* const Fa = lists;
     uint32_t* const Fb = lists + n;
-    uint32_t* const Pa = lists + 2 * static_cast<size_t>(n);
-    uint32_t* const Pb = lists + 3 * static_cast<size_t>(n);
-    uint32_t* const Sl = lists + 4 * static_cast<size_t>(n);
+    uint32_t* const Sl = lists + 2 * static_cast<size_t>(n);
+    const uint32_t pcap = a.pcap;
+    PileEntry<B>* const Pa = static_cast<PileEntry<B>*>(a.pile) + static_cast<size_t>(slot) * 2 * pcap;
+    PileEntry<B>* const Pb = Pa + pcap;
     double* sigma = nullptr;
     double* delta1 = nullptr;
     double* delta2 = nullptr;
@@ -438,7 +470,6 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
             bsync();
         }
         ++gen;
-        ++pgen;
         ++phase;
         if (use_bm)
             for (uint32_t w = tid; w < a.bm_words; w += kBlock) bm[w] = 0u;
@@ -452,8 +483,8 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
         }
         uint32_t* Fc = Fa;
         uint32_t* Fn = Fb;
-        uint32_t* Pc = Pa;
-        uint32_t* Pn = Pb;
+        PileEntry<B>* Pc = Pa;
+        PileEntry<B>* Pn = Pb;
         uint32_t nF = 1;
         uint32_t settle_from = 0;
         B hi = T::sat_add(B(0), delta);
@@ -494,12 +525,21 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                         pf = wg_exch(&fstamp[t], nph) != nph;
                     } else {
                         atomicMin(&S.runmin, nd);
-                        pp = wg_exch(&pstamp[t], pgen) != pgen;
+                        // first parking of t in this source always fits (the pile keeps n entries in reserve);
+                        // a re-parking with the improved value only while the reserve is untouched, else the
+                        // next bucket advance reloads the distances (exact mode)
+                        if (wg_exch(&pstamp[t], gen) != gen) {
+                            pp = true;
+                        } else if (S.cnt[1] + kBlock < pcap - n) {
+                            pp = true;
+                        } else {
+                            S.cnt[5] = 1u;
+                        }
                     }
                 }
             }
             push(pf, t, Fn, &S.cnt[0], n);
-            push(pp, t, Pc, &S.cnt[1], n);
+            push_entry(pp, nd, t, Pc, &S.cnt[1], pcap);
             if constexpr (MODE != MODE_SSSP) {
                 const bool lg = ok && log_it_if_close && !final_t && nd <= cur;
                 const unsigned long long mask = __ballot(lg);
@@ -597,8 +637,9 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                     continue;
                 }
             }
-            const uint32_t nP = clampn(S.cnt[1]);
+            const uint32_t nP = S.cnt[1] > pcap ? (clampn(n + 1), 0u) : S.cnt[1];
             if (aborted || nP == 0) break;
+            const bool exact = S.cnt[5] != 0u;
             if (++guard > guard_max) {
                 if (tid == 0) atomicOr(a.err, ERR_GUARD);
                 aborted = true;
@@ -613,34 +654,38 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
             // lo + delta may round back to lo for real-valued distances: the window must not be empty,
             // or the pile would never drain. The next representable value is bits + 1.
             if (hi <= lo && lo != T::kInf) hi = lo + 1;
-            ++pgen;
             const uint32_t nph = phase + 1;
             bsync();
             if (tid == 0) {
                 S.runmin = T::kInf;
                 S.cnt[0] = 0;
                 S.cnt[1] = 0;
+                S.cnt[5] = 0;
             }
             bsync();
+            // split the pile: entries of settled nodes are dropped, values below hi join the frontier
+            // (duplicates removed by the frontier stamp), the rest is carried over with its value
             for (uint32_t base = 0; base < nP; base += kBlock) {
                 const uint32_t i = base + tid;
                 bool tf = false, tp = false;
                 uint32_t t = 0;
+                B d = 0;
                 if (i < nP) {
-                    t = Pc[i];
+                    const PileEntry<B> pe = Pc[i];
+                    t = pe.node;
                     const bool settled = use_bm ? ((bm[t >> 5] >> (t & 31)) & 1u) != 0u : ldc(&sstamp[t]) == gen;
                     if (!settled) {
-                        const B d = ldc(&dist[t]);
+                        d = exact ? ldc(&dist[t]) : pe.val;
                         if (d < hi) {
                             tf = wg_exch(&fstamp[t], nph) != nph;
                         } else {
                             atomicMin(&S.runmin, d);
-                            tp = wg_exch(&pstamp[t], pgen) != pgen;
+                            tp = true;
                         }
                     }
                 }
                 push(tf, t, Fn, &S.cnt[0], n);
-                push(tp, t, Pn, &S.cnt[1], n);
+                push_entry(tp, d, t, Pn, &S.cnt[1], pcap);
             }
             bsync();
             nF = clampn(S.cnt[0]);
@@ -649,9 +694,9 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                 uint32_t* tmp = Fc;
                 Fc = Fn;
                 Fn = tmp;
-                tmp = Pc;
+                PileEntry<B>* ptmp = Pc;
                 Pc = Pn;
-                Pn = tmp;
+                Pn = ptmp;
             }
             phase = nph;
             bsync();
@@ -951,6 +996,8 @@ hipError_t launch_typed(const LaunchArgs& la, hipStream_t stream) {
     k.sstamp = ws.sstamp;
     k.pstamp = ws.pstamp;
     k.lists = ws.lists;
+    k.pile = ws.pile;
+    k.pcap = ws.pcap;
     k.counters = ws.counters;
     k.sigma = ws.sigma;
     k.delta1 = ws.delta1;

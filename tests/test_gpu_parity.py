@@ -229,3 +229,18 @@ def test_fixed_degree_config_parity(ctx):
     assert agg.number_disconnected_pairs == ref["n_disconnected"]
     assert agg.cluster_sizes == ref["cluster_sizes"]
     assert agg.average_efficiency == pytest.approx(ref["avg_eff"], rel=1e-12)
+
+
+def test_small_pile_forces_exact_bucket_advance(ctx, monkeypatch):
+    """With the pile reserve exhausted, bucket advances reload distances instead of trusting parked values."""
+    monkeypatch.setenv("NG_PILE_CAPACITY", "1")  # clamped to n + 512 entries: every re-parking is refused
+    for wdtype in (np.int32, np.float32):
+        c = rand_graph(3000, 30000, 77, wdtype=wdtype, wlo=1, whi=20)
+        g = dev_graph(ctx, c)
+        srcs = np.arange(0, 3000, 37)
+        ref = np.stack([orc.sssp(c, int(s))[0] for s in srcs])
+        for delta in (0.0, 0.5, 3.0):
+            _, dist = g.sssp(srcs, delta=delta, want_dist=True)
+            np.testing.assert_array_equal(dist, ref)
+        bc = g.brandes(srcs)
+        np.testing.assert_allclose(bc, orc.brandes(c, sources=srcs)["bc"], rtol=1e-9, atol=1e-9)
