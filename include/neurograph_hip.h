@@ -75,6 +75,10 @@ typedef struct {
     uint64_t layers;         /* Kahn layers of the tight DAG (betweenness modes) */
     uint64_t probes;         /* arc-target distance probes issued (arcs minus skipped ones) */
     uint64_t improvements;   /* successful distance improvements */
+    uint64_t section_ticks[8]; /* wall-clock ticks per kernel section (frontier phases, settle passes, bucket
+                                  advances, summaries, tight-arc verification, sigma layers, dependency
+                                  layers, slot reset), summed over workgroups; zero unless the library was
+                                  built with -DNG_SECTION_TIMERS */
 } ng_call_stats;
 
 typedef struct ng_context ng_context;

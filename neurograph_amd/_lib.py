@@ -46,7 +46,8 @@ class CallStats(ctypes.Structure):
                 ("delta", ctypes.c_double), ("arcs_reached", ctypes.c_uint64), ("flags", ctypes.c_uint32),
                 ("reserved", ctypes.c_uint32), ("phases", ctypes.c_uint64), ("buckets", ctypes.c_uint64),
                 ("light_arcs", ctypes.c_uint64), ("settle_arcs", ctypes.c_uint64), ("logged_arcs", ctypes.c_uint64),
-                ("layers", ctypes.c_uint64), ("probes", ctypes.c_uint64), ("improvements", ctypes.c_uint64)]
+                ("layers", ctypes.c_uint64), ("probes", ctypes.c_uint64), ("improvements", ctypes.c_uint64),
+                ("section_ticks", ctypes.c_uint64 * 8)]
 
 
 class HostCsrC(ctypes.Structure):
@@ -298,7 +299,9 @@ class Graph:
     def last_stats(self) -> dict:
         s = CallStats()
         _check(lib().ng_graph_last_stats(self.h, ctypes.byref(s)))
-        return {f: getattr(s, f) for f, _ in CallStats._fields_}
+        out = {f: getattr(s, f) for f, _ in CallStats._fields_}
+        out["section_ticks"] = list(s.section_ticks)
+        return out
 
 
 def version() -> str:

@@ -357,7 +357,7 @@ ng_status ensure_ws(ng_graph* g, uint32_t slots, int mode, DKind dk, uint32_t n_
             NG_HIP(launch_fill_u32(w.indeg, ns, 0u, st));
             NG_HIP(launch_fill_u32(w.bits, w.words * slots, 0u, st));
         }
-        NG_HIP(hipMalloc(&w.arcs_reached, 8 * 16));
+        NG_HIP(hipMalloc(&w.arcs_reached, 8 * 32));
         NG_HIP(hipMalloc(&w.err, 4));
         g->ws_dist_valid = false;
     }
@@ -440,7 +440,7 @@ ng_status run_batch(ng_graph* g, const RunSpec& rs, uint32_t* slots_used, double
     Workspace& w = g->ws;
     const hipStream_t s = ctx->stream;
     if (rs.n_sources) NG_HIP(hipMemcpyAsync(w.sources, rs.sources, sizeof(uint32_t) * rs.n_sources, hipMemcpyHostToDevice, s));
-    NG_HIP(hipMemsetAsync(w.arcs_reached, 0, 8 * 16, s));
+    NG_HIP(hipMemsetAsync(w.arcs_reached, 0, 8 * 32, s));
     NG_HIP(hipMemsetAsync(w.err, 0, 4, s));
     if (rs.mode & MODE_BC) NG_HIP(launch_fill_f64(w.bc_slot, static_cast<uint64_t>(slots) * g->n, 0.0, s));
     if (rs.mode & MODE_ARC1) NG_HIP(launch_fill_f64(w.grad1, g->m, 0.0, s));
@@ -478,9 +478,10 @@ ng_status run_batch(ng_graph* g, const RunSpec& rs, uint32_t* slots_used, double
     }
     if (rs.n_sources) NG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     uint32_t err = 0;
-    unsigned long long ctrs[16] = {};
+    unsigned long long ctrs[32] = {};
     NG_HIP(hipMemcpy(&err, w.err, 4, hipMemcpyDeviceToHost));
-    NG_HIP(hipMemcpy(ctrs, w.arcs_reached, 8 * 16, hipMemcpyDeviceToHost));
+    NG_HIP(hipMemcpy(ctrs, w.arcs_reached, 8 * 32, hipMemcpyDeviceToHost));
+    for (int sct = 0; sct < 8; ++sct) g->last.section_ticks[sct] = ctrs[9 + sct];
     const unsigned long long arcs = ctrs[0];
     if (rs.dist_out) {
         NG_HIP(hipMemcpy(rs.dist_out, d_dist_out, dob * static_cast<size_t>(rs.n_sources) * g->n, hipMemcpyDeviceToHost));

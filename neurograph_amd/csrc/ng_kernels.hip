@@ -404,6 +404,19 @@ __device__ __forceinline__ uint32_t range_mask(uint32_t wi, uint32_t b, uint32_t
     return m;
 }
 
+// Optional per-section wall-clock accounting (compile with -DNG_SECTION_TIMERS): workgroup-uniform
+// timestamps, summed by lane 0 into arcs_reached[8 + section]. Off in the normal build.
+#ifdef NG_SECTION_TIMERS
+#define NG_TICK(var) const unsigned long long var = wall_clock64()
+#define NG_TOCK(var, sec) \
+    do { \
+        if (threadIdx.x == 0) atomicAdd(&a.arcs_reached[9 + (sec)], wall_clock64() - (var)); \
+    } while (0)
+#else
+#define NG_TICK(var)
+#define NG_TOCK(var, sec)
+#endif
+
 template <class D, int MODE>
 __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
     using T = DT<D>;
@@ -565,6 +578,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                 }
                 const uint32_t nph = phase + 1;
                 ++c_phases;
+                NG_TICK(t_inner);
                 expand_relax<NG_UNROLL>(
                     Fc, nF, S, a.tgt, a.len,
                     [&](uint32_t v, B& dv, uint32_t& beg, uint32_t& cnt) {
@@ -602,12 +616,14 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                 bsync();
                 if (tid == 0) S.cnt[0] = 0;
                 bsync();
+                NG_TOCK(t_inner, 0);
             }
             if (aborted) break;
             // ---- settle pass: every settled node's row once, with its final distance
             const uint32_t nS = clampn(S.cnt[2]);
             if (aborted) break;
             if (nS > settle_from) {
+                NG_TICK(t_settle);
                 const uint32_t nph = phase + 1;
                 expand_relax<NG_UNROLL>(
                     Sl + settle_from, nS - settle_from, S, a.tgt, a.len,
@@ -622,6 +638,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                         apply(ok, v, dv, e, t, l, fin, cur, nph, true);
                     });
                 bsync();
+                NG_TOCK(t_settle, 1);
                 settle_from = nS;
                 nF = clampn(S.cnt[0]);
                 if (aborted) break;
@@ -646,6 +663,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                 break;
             }
             // ---- next bucket: [max(pile minimum, hi), +delta)
+            NG_TICK(t_split);
             ++c_buckets;
             const B rm = S.runmin;
             const B lo = rm > hi ? rm : hi;
@@ -702,8 +720,10 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
             bsync();
             if (tid == 0) S.cnt[0] = 0;
             bsync();
+            NG_TOCK(t_split, 2);
         }
 
+        NG_TICK(t_summary);
         const uint32_t nS = S.cnt[2] > n ? n : S.cnt[2];
         // ---- per-source summary (SsspResult.h:43-64) over the reached nodes
         {
@@ -764,7 +784,9 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
             }
         }
 
+        NG_TOCK(t_summary, 3);
         if constexpr (MODE != MODE_SSSP) if (!aborted) {
+            NG_TICK(t_verify);
             // ---- tight arcs: verify the logged candidates against the final distances
             const uint32_t nlog = S.cnt[3];
             c_logged += nlog;
@@ -793,6 +815,8 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                     [&](uint32_t v, B, uint32_t e) { mark(v, e); });
             }
             bsync();
+            NG_TOCK(t_verify, 4);
+            NG_TICK(t_kahn);
             // ---- Kahn layers over the tight DAG: sigma and a topological order
             if (tid == 0) {
                 order[0] = s;
@@ -842,6 +866,8 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                 }
             }
             bsync();
+            NG_TOCK(t_kahn, 5);
+            NG_TICK(t_back);
             // ---- reverse layers: dependencies, per-node and per-arc accumulation, bitmap cleared
             for (int ly = static_cast<int>(nl) - 1; ly >= 0; --ly) {
                 const uint32_t l0 = layers[ly], l1 = layers[ly + 1];
@@ -880,8 +906,10 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                 }
                 bsync();
             }
+            NG_TOCK(t_back, 6);
         }
 
+        NG_TICK(t_reset);
         if (a.dist_out != nullptr) {
             for (uint32_t v = tid; v < n; v += kBlock)
                 write_dist_out(a, static_cast<size_t>(si) * n + v, ldc(&dist[v]));
@@ -902,6 +930,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
             }
         }
         bsync();
+        NG_TOCK(t_reset, 7);
     }
     if (tid == 0) {
         ctr[0] = phase;
