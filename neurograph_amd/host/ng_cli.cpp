@@ -1,0 +1,166 @@
+// Command-line front end of the MI355X path, with the reference's flags for the path metrics
+// (source/neurograph.cpp:449-778) and MetricStore-compatible output files
+// (source/utility/MetricStore.cpp:284-373), written to <input>/metrics[/<output-directory>].
+//
+//   ng_cli --input DIR --func avg-apsp,avg-apsp-inv,avg-centrality [--apsp-algorithm dijkstra|delta-stepping]
+//          [--apsp-delta D] [--apsp-delta-epoch push-check|...|gpu] [--dijkstra-queue priority-queue|...|gpu]
+//          [--centrality-algorithm brandes|path-enumeration|gpu] [--undirected] [--all-weights-one]
+//          [--remove-autapses] [--prefix P] [--output-directory D] [--threads N]
+//          [--weights int|f32|f64] [--device N] [--node-betweenness]
+//
+// Every selector value runs the GPU variant (they compute identical values); the reference's refusal of
+// the bucket queue with avg-apsp-inv is kept (neurograph.cpp:690-693). --weights f32|f64 reads real-valued
+// weights (extension); --node-betweenness also writes per-node values (extension).
+
+#include <algorithm>
+#include <charconv>
+#include <cstdio>
+#include <cstring>
+#include <filesystem>
+#include <fstream>
+#include <iostream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "ng_variants.hpp"
+
+namespace fs = std::filesystem;
+
+namespace {
+
+// Shortest round-trip text of a double, like {fmt}'s "{}".
+std::string num(double v) {
+    char buf[64];
+    auto r = std::to_chars(buf, buf + sizeof buf, v);
+    return std::string(buf, r.ptr);
+}
+std::string num(std::uint64_t v) { return std::to_string(v); }
+
+std::vector<std::string> split(const std::string& s) {
+    std::vector<std::string> out;
+    std::stringstream ss(s);
+    for (std::string item; std::getline(ss, item, ',');)
+        if (!item.empty()) out.push_back(item);
+    return out;
+}
+
+template <typename R>
+std::string apsp_text(const R& r) {
+    std::map<std::uint64_t, std::uint64_t> sorted(r.cluster_sizes.begin(), r.cluster_sizes.end());
+    std::ostringstream ss;
+    for (const auto& [size, occ] : sorted) ss << "Size: " << size << "\tOccurences: " << occ << "\n";
+    ss << "The average shortest path is: " << num(r.average_shortest_path_length) << "\n";
+    ss << "The global efficiency is: " << num(r.average_efficiency) << "\n";
+    ss << "The diameter of the graph is: " << num(r.diameter) << "\n";
+    ss << "However, the number of pairs without path is: " << r.number_disconnected_pairs << "\n";
+    ss << "The clusters were distributed as follows:\n";
+    for (const auto& [size, occ] : sorted) ss << "Size: " << size << "\tOccurences: " << occ << "\n";
+    return ss.str();
+}
+
+void emit(const fs::path& dir, const std::string& file, const std::string& text) {
+    std::cout << text;
+    std::ofstream(dir / file) << text;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    std::map<std::string, std::string> opt;
+    std::vector<std::string> flags;
+    for (int i = 1; i < argc; ++i) {
+        std::string a = argv[i];
+        if (a == "--undirected" || a == "--all-weights-one" || a == "--remove-autapses" || a == "--node-betweenness" ||
+            a == "--disable-status-report" || a == "--disable-timer-report") {
+            flags.push_back(a);
+        } else if (a.rfind("--", 0) == 0 && i + 1 < argc) {
+            opt[a] = argv[++i];
+        } else {
+            std::cerr << "unknown argument " << a << "\n";
+            return EXIT_FAILURE;
+        }
+    }
+    auto has = [&](const char* f) { return std::find(flags.begin(), flags.end(), f) != flags.end(); };
+    auto get = [&](const char* k, const char* d) { return opt.count(k) ? opt[k] : std::string(d); };
+    if (!opt.count("--input")) {
+        std::cerr << "--input is required\n";
+        return EXIT_FAILURE;
+    }
+    const fs::path input = opt["--input"];
+    bool apsp = false, apsp_inv = false, centrality = false;
+    for (const auto& f : split(get("--func", ""))) {
+        if (f == "avg-apsp") apsp = true;
+        else if (f == "avg-apsp-inv") apsp_inv = true;
+        else if (f == "avg-centrality") centrality = true;
+        else {
+            std::cerr << "The argument \"" << f << "\" is not a valid algorithm of this build (avg-apsp, avg-apsp-inv, avg-centrality)\n";
+            return EXIT_FAILURE;
+        }
+    }
+    const std::string algo = get("--apsp-algorithm", "dijkstra");
+    const std::string queue = get("--dijkstra-queue", "gpu");
+    const std::string epoch = get("--apsp-delta-epoch", "gpu");
+    const std::string calgo = get("--centrality-algorithm", "gpu");
+    if (algo != "dijkstra" && algo != "delta-stepping") {
+        std::cerr << "The argument \"" << algo << "\" is not a valid --apsp-algorithm\n";
+        return EXIT_FAILURE;
+    }
+    for (const auto& [val, valid] : std::vector<std::pair<std::string, std::vector<std::string>>>{
+             {queue, {"priority-queue", "fibonacci-heap", "bucket-queue", "gpu"}},
+             {epoch, {"push-brute-force", "push-check", "push-short-long", "push-inner-short", "pull-model", "gpu"}},
+             {calgo, {"brandes", "path-enumeration", "gpu"}}}) {
+        if (std::find(valid.begin(), valid.end(), val) == valid.end()) {
+            std::cerr << "The argument \"" << val << "\" is not a valid selector\n";
+            return EXIT_FAILURE;
+        }
+    }
+    if (apsp_inv && queue == "bucket-queue") {
+        std::cerr << "--dijkstra-queue bucket-queue cannot be used with avg-apsp-inv: the inverse distances are not integers\n";
+        return EXIT_FAILURE;
+    }
+    const std::string wk = get("--weights", "int");
+    const ng_weight_kind wkind = wk == "f64" ? NG_W_F64 : (wk == "f32" ? NG_W_F32 : NG_W_I32);
+    fs::path out_dir = input / "metrics";
+    if (opt.count("--output-directory")) out_dir /= opt["--output-directory"];
+
+    try {
+        ng_host_csr csr{};
+        ng_host::check(ng_load_graph_dir(input.c_str(), get("--prefix", "").c_str(), 0, has("--remove-autapses"), has("--undirected"),
+                                         has("--all-weights-one"), wkind, &csr),
+                       "loading the graph");
+        ng_host::Context ctx(std::stoi(get("--device", "0")));
+        ng_host::Graph graph(ctx, csr);
+        ng_host_csr_free(&csr);
+        std::cout << "Loaded " << graph.number_nodes() << " nodes and " << graph.number_arcs() << " arcs\n";
+        fs::create_directories(out_dir);
+        if (apsp) {
+            if (wkind == NG_W_I32) {
+                const auto r = algo == "delta-stepping"
+                                   ? ng_host::AllPairsShortestPath::compute_apsp_delta_stepping(graph, std::stoull(get("--apsp-delta", "1")))
+                                   : ng_host::AllPairsShortestPath::compute_apsp(graph);
+                emit(out_dir, "all_pairs_shortest_paths.txt", apsp_text(r));
+            } else {
+                emit(out_dir, "all_pairs_shortest_paths.txt", apsp_text(ng_host::AllPairsShortestPath::compute_apsp_real(graph)));
+            }
+        }
+        if (apsp_inv) {
+            emit(out_dir, "all_pairs_shortest_paths_inverse.txt", apsp_text(ng_host::AllPairsShortestPath::compute_apsp_inverse(graph)));
+        }
+        if (centrality) {
+            std::vector<double> per_node;
+            const double avg = ng_host::BetweennessCentrality::compute_average_betweenness_centrality(
+                graph, ng_host::BetweennessCentralityAlgorithm::Gpu, &per_node);
+            emit(out_dir, "average_betweenness_centrality.txt", "The average betweenness centrality is: " + num(avg) + "\n");
+            if (has("--node-betweenness")) {
+                std::ofstream f(out_dir / "node_betweenness_centrality.txt");
+                for (std::size_t v = 0; v < per_node.size(); ++v) f << (v + 1) << "\t" << num(per_node[v]) << "\n";
+            }
+        }
+    } catch (const std::exception& e) {
+        std::cerr << e.what() << "\n";
+        return EXIT_FAILURE;
+    }
+    return EXIT_SUCCESS;
+}

@@ -1,0 +1,94 @@
+"""The command-line front end (neurograph_amd/ng_cli): reference flags in, MetricStore-format files out
+(source/utility/MetricStore.cpp:284-373), values checked against the reference's assertions and the oracle."""
+
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from oracle import oracle as orc
+from tests.conftest import csr_from_fixture
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "neurograph_amd", "ng_cli")
+
+
+def run_cli(*args):
+    r = subprocess.run([CLI, *args], capture_output=True, text=True, timeout=120)
+    return r
+
+
+def parse_apsp(path):
+    text = open(path).read()
+    avg = float(re.search(r"The average shortest path is: (\S+)", text).group(1))
+    eff = float(re.search(r"The global efficiency is: (\S+)", text).group(1))
+    diam = float(re.search(r"The diameter of the graph is: (\S+)", text).group(1))
+    disc = int(re.search(r"number of pairs without path is: (\d+)", text).group(1))
+    tail = text.split("The clusters were distributed as follows:\n")[1]
+    clusters = {int(a): int(b) for a, b in re.findall(r"Size: (\d+)\tOccurences: (\d+)", tail)}
+    return avg, eff, diam, disc, clusters
+
+
+def write_fixture(tmp_path, fx):
+    import neurograph_amd as ng
+
+    c = csr_from_fixture(fx)
+    d = str(tmp_path / "input")
+    ng.write_graph_dir(d, ng.HostGraph(c.n, c.off, c.tgt, c.w))
+    return d
+
+
+@pytest.mark.parametrize("name", ["standard", "seven"])
+def test_cli_matches_reference_assertions(tmp_path, reference_fixtures, name):
+    fx = reference_fixtures["fixtures"][name]
+    d = write_fixture(tmp_path, fx)
+    r = run_cli("--input", d, "--func", "avg-apsp,avg-apsp-inv,avg-centrality")
+    assert r.returncode == 0, r.stderr
+    out = os.path.join(d, "metrics")
+    avg, eff, diam, disc, clusters = parse_apsp(os.path.join(out, "all_pairs_shortest_paths.txt"))
+    exp = fx["apsp"]
+    assert avg == pytest.approx(exp["avg_spl"], abs=1e-12)
+    assert eff == pytest.approx(exp["avg_eff"], abs=1e-6)
+    assert diam == exp["diameter"] and disc == exp["n_disconnected"]
+    assert clusters == {int(k): v for k, v in exp["cluster_sizes"].items()}
+    avg, eff, diam, disc, _ = parse_apsp(os.path.join(out, "all_pairs_shortest_paths_inverse.txt"))
+    assert avg == pytest.approx(fx["apsp_inverse"]["avg_spl"], abs=1e-6)
+    assert diam == pytest.approx(fx["apsp_inverse"]["diameter"], abs=1e-6)
+    text = open(os.path.join(out, "average_betweenness_centrality.txt")).read()
+    bc = float(re.search(r"The average betweenness centrality is: (\S+)", text).group(1))
+    assert bc == pytest.approx(fx["brandes_avg"]["value"], abs=1e-6)
+
+
+def test_cli_selectors_and_errors(tmp_path, reference_fixtures):
+    d = write_fixture(tmp_path, reference_fixtures["fixtures"]["standard"])
+    r = run_cli("--input", d, "--func", "avg-apsp-inv", "--dijkstra-queue", "bucket-queue")
+    assert r.returncode != 0 and "bucket-queue" in r.stderr  # neurograph.cpp:690-693
+    r = run_cli("--input", d, "--func", "avg-apsp", "--apsp-algorithm", "delta-stepping", "--apsp-delta", "3",
+                "--apsp-delta-epoch", "pull-model", "--output-directory", "ds")
+    assert r.returncode == 0, r.stderr
+    avg, *_ = parse_apsp(os.path.join(d, "metrics", "ds", "all_pairs_shortest_paths.txt"))
+    assert avg == 3.1
+    r = run_cli("--input", d, "--func", "modularity")
+    assert r.returncode != 0
+
+
+def test_cli_real_weights_and_node_output(tmp_path):
+    import neurograph_amd as ng
+
+    hg = ng.generate_spatial(3000, 12, seed=9, weights=np.float64)
+    d = str(tmp_path / "sp")
+    ng.write_graph_dir(d, hg)
+    r = run_cli("--input", d, "--func", "avg-apsp,avg-centrality", "--weights", "f64", "--node-betweenness")
+    assert r.returncode == 0, r.stderr
+    c = orc.Csr(hg.n, hg.offsets, hg.targets, hg.weights)
+    ref = orc.apsp(c, threads=8)
+    avg, eff, diam, disc, clusters = parse_apsp(os.path.join(d, "metrics", "all_pairs_shortest_paths.txt"))
+    assert avg == pytest.approx(ref["avg_spl"], rel=1e-12)
+    assert diam == ref["diameter"]
+    assert clusters == ref["cluster_sizes"]
+    per = np.loadtxt(os.path.join(d, "metrics", "node_betweenness_centrality.txt"))
+    np.testing.assert_allclose(per[:, 1], orc.brandes(c, threads=8)["bc"], rtol=1e-9, atol=1e-9)
