@@ -68,7 +68,7 @@ def cpu_baseline(g, sources, seconds: float):
 
     c = orc.Csr(g.n, g.offsets, g.targets, g.weights)
     threads = max(1, min(16, os.cpu_count() or 1))
-    probe = sources[: max(threads, 1)]
+    probe = sources[: 2 * threads]
     t0 = time.perf_counter()
     orc.apsp(c, sources=probe, threads=threads)
     orc.brandes(c, sources=probe, threads=threads)
@@ -95,7 +95,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="spatial50k", choices=["fixed5k", "spatial50k", "spatial200k"])
     ap.add_argument("--batch", type=int, default=0)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--delta", type=float, default=0.0, help="bucket width (0: autotune)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -148,6 +148,11 @@ def main():
     wall = time.perf_counter() - t0
 
     value = srcs / wall
+    if args.copy_calibration_mib > 0:
+        nb = args.copy_calibration_mib << 20
+        cms = ctx.copy_calibration(nb)
+        print(f"[bench] copy_kernel {nb} bytes read + {nb} written in {cms:.3f} ms "
+              f"({2 * nb / cms / 1e6:.1f} GB/s)", file=sys.stderr, flush=True)
     launches_bytes = [bytes_per_source(hg) * len(batches[k % len(batches)]) for k in range(steps)]
     achieved = sum(launches_bytes) / (sum(kernel_ms) / 1e3) / 1e9
     traffic = None
