@@ -6,6 +6,7 @@ from ._lib import (  # noqa: F401
     NG_LEN_ABS,
     NG_LEN_INVERSE,
     NG_LEN_RAW,
+    Cohort,
     Context,
     Graph,
     HostGraph,

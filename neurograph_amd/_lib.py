@@ -86,6 +86,8 @@ def lib() -> ctypes.CDLL:
         "ng_cohort_create": (i32, [vp, vp, u32, P(vp)]),
         "ng_cohort_destroy": (None, [vp]),
         "ng_cohort_brandes": (i32, [vp, vp, vp, P(dbl)]),
+        "ng_cohort_autotune": (i32, [vp, u32]),
+        "ng_cohort_last_stats": (i32, [vp, P(CallStats)]),
         "ng_last_error": (ctypes.c_char_p, []),
         "ng_version": (ctypes.c_char_p, []),
         "ng_host_csr_free": (None, [P(HostCsrC)]),
@@ -299,6 +301,52 @@ class Graph:
     def last_stats(self) -> dict:
         s = CallStats()
         _check(lib().ng_graph_last_stats(self.h, ctypes.byref(s)))
+        out = {f: getattr(s, f) for f, _ in CallStats._fields_}
+        out["section_ticks"] = list(s.section_ticks)
+        return out
+
+
+class Cohort:
+    """All sources of many graphs in one persistent, hipGraph-captured launch (ng_cohort_*). The member
+    Graph objects must stay alive while the cohort is used."""
+
+    def __init__(self, ctx: Context, graphs):
+        self.ctx = ctx
+        self.graphs = list(graphs)
+        arr = (ctypes.c_void_p * len(self.graphs))(*[g.h.value for g in self.graphs])
+        h = ctypes.c_void_p()
+        _check(lib().ng_cohort_create(ctx.h, arr, len(self.graphs), ctypes.byref(h)))
+        self.h = h
+        self.sizes = [g.n for g in self.graphs]
+        self.prefix = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
+
+    def close(self):
+        if self.h:
+            lib().ng_cohort_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def autotune(self, n_sample: int = 256):
+        _check(lib().ng_cohort_autotune(self.h, n_sample))
+
+    def brandes(self, summaries: bool = False):
+        """Returns (list of per-graph betweenness vectors, summaries or None, replay ms)."""
+        total = int(self.prefix[-1])
+        bc = np.zeros(total, dtype=np.float64)
+        out = np.zeros(total, dtype=SUMMARY_DTYPE) if summaries else None
+        ms = ctypes.c_double()
+        _check(lib().ng_cohort_brandes(self.h, _ptr(bc), _ptr(out), ctypes.byref(ms)))
+        parts = [bc[self.prefix[i]:self.prefix[i + 1]] for i in range(len(self.graphs))]
+        return parts, out, ms.value
+
+    def last_stats(self) -> dict:
+        s = CallStats()
+        _check(lib().ng_cohort_last_stats(self.h, ctypes.byref(s)))
         out = {f: getattr(s, f) for f, _ in CallStats._fields_}
         out["section_ticks"] = list(s.section_ticks)
         return out

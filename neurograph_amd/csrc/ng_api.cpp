@@ -98,14 +98,36 @@ struct ng_graph {
     bool has_negative = false;
     View views[2];                 // [0] |w|, [1] 1/|w|
     Workspace ws;
-    DKind ws_dist_kind = DKind::U32;
-    bool ws_dist_valid = false;
     ng_call_stats last{};
 };
 
+// Cohort: all sources of many graphs in one persistent launch, captured once in a hipGraph and replayed.
 struct ng_cohort {
     ng_context* ctx = nullptr;
     std::vector<ng_graph*> graphs;
+    DKind dk = DKind::U32;
+    uint32_t n_max = 0;
+    uint64_t m_max = 0;
+    uint32_t total = 0;                // sum of member sizes = number of (graph, source) items
+    std::vector<uint32_t> prefix;      // [G+1] item / node prefix per graph
+    std::vector<double> delta;         // per graph (0: the graph's tuned or heuristic width)
+    std::vector<double> cost;          // per-source cost weight per graph (balances the static partition)
+    int bpc = 0;                       // resident workgroups per CU (0: occupancy maximum)
+    uint32_t slots = 0;
+    Workspace ws;
+    CohortGraph* d_gtab = nullptr;
+    uint32_t* d_lend = nullptr;        // light-prefix ends of every member for its own width
+    uint32_t* d_prefix = nullptr;
+    uint32_t* d_chunk = nullptr;
+    uint64_t* d_fbase = nullptr;
+    uint32_t* d_seg_ptr = nullptr;
+    uint64_t* d_seg_off = nullptr;
+    double* d_flush = nullptr;
+    double* d_bc = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool built = false;
+    ng_call_stats last{};
 };
 
 namespace {
@@ -287,6 +309,14 @@ void free_ws(Workspace& w) {
 }
 
 // Pile capacity per list: n first parkings plus 2n re-parkings with improved values.
+// Settle passes expand each bucket's settled nodes in increasing id (a second LDS bitmap) when that
+// costs no resident workgroups: measured +5 % on a 50k-node spatial graph, but a loss wherever the larger
+// LDS footprint lowers occupancy. NG_SORTED_SETTLE=0/1 overrides the choice (tuning experiments).
+bool settle_sorted(DKind dk, int mode, uint32_t n) {
+    if (const char* e = std::getenv("NG_SORTED_SETTLE")) return std::atoi(e) != 0;
+    return batch_blocks_per_cu(dk, mode, n, true) >= batch_blocks_per_cu(dk, mode, n, false);
+}
+
 uint32_t pile_capacity(uint32_t n) { return static_cast<uint32_t>(std::min<uint64_t>(3ull * n + 1024, 0xFFFFFF00u)); }
 
 // Per-slot device bytes of the batched kernel for (n, m, mode).
@@ -303,16 +333,13 @@ uint32_t log_capacity(uint32_t n, uint64_t m) {
     return static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>(m, 1), want));
 }
 
-// Makes sure the workspace holds `slots` slots of every array `mode` needs, with dist in kind dk.
-ng_status ensure_ws(ng_graph* g, uint32_t slots, int mode, DKind dk, uint32_t n_sources) {
-    Workspace& w = g->ws;
-    const hipStream_t st = g->ctx->stream;
+// Makes sure workspace w holds `slots` slots for graphs of up to n nodes and m arcs, with every array `mode`
+// needs and dist in kind dk.
+ng_status ensure_ws(Workspace& w, hipStream_t st, uint32_t n, uint64_t m, uint32_t slots, int mode, DKind dk, uint32_t n_sources) {
     const int need_bits = (mode == MODE_SSSP) ? 0 : (MODE_BC | MODE_ARC1 | MODE_ARC2);
-    const bool realloc = w.slots < slots || w.n != g->n || dk_size(dk) > w.dist_bytes || (need_bits & ~w.mode_bits) != 0;
+    const bool realloc = w.slots < slots || w.n != n || w.m < m || dk_size(dk) > w.dist_bytes || (need_bits & ~w.mode_bits) != 0;
     if (realloc) {
         free_ws(w);
-        const uint32_t n = g->n;
-        const uint64_t m = g->m;
         const size_t ns = static_cast<size_t>(slots) * n;
         w.slots = slots;
         w.n = n;
@@ -359,12 +386,12 @@ ng_status ensure_ws(ng_graph* g, uint32_t slots, int mode, DKind dk, uint32_t n_
         }
         NG_HIP(hipMalloc(&w.arcs_reached, 8 * 32));
         NG_HIP(hipMalloc(&w.err, 4));
-        g->ws_dist_valid = false;
+        w.dist_valid = false;
     }
-    if (!g->ws_dist_valid || g->ws_dist_kind != dk) {
-        NG_HIP(launch_fill_dist(dk, w.dist, static_cast<uint64_t>(w.slots) * g->n, st));
-        g->ws_dist_kind = dk;
-        g->ws_dist_valid = true;
+    if (!w.dist_valid || w.dist_kind != dk) {
+        NG_HIP(launch_fill_dist(dk, w.dist, static_cast<uint64_t>(w.slots) * w.n, st));
+        w.dist_kind = dk;
+        w.dist_valid = true;
     }
     if (w.summaries_cap < n_sources) {
         if (w.summaries) (void)hipFree(w.summaries);
@@ -420,7 +447,8 @@ ng_status run_batch(ng_graph* g, const RunSpec& rs, uint32_t* slots_used, double
         v.lend_delta = delta;
     }
     // slots: resident workgroups of this kernel, bounded by the sources and by device memory
-    const int bpc = batch_blocks_per_cu(v.dk, rs.mode, g->n);
+    const bool sorted = settle_sorted(v.dk, rs.mode, g->n);
+    const int bpc = batch_blocks_per_cu(v.dk, rs.mode, g->n, sorted);
     uint64_t slots = static_cast<uint64_t>(bpc) * static_cast<uint64_t>(ctx->cus);
     // NG_MAX_SLOTS caps the concurrent source slots (for tuning sweeps and experiments)
     if (const char* cap = std::getenv("NG_MAX_SLOTS")) {
@@ -435,7 +463,7 @@ ng_status run_batch(ng_graph* g, const RunSpec& rs, uint32_t* slots_used, double
     const uint64_t held = static_cast<uint64_t>(g->ws.slots) * per;
     const uint64_t budget = static_cast<uint64_t>(static_cast<double>(free_b + held) * 0.6);
     if (per > 0) slots = std::min<uint64_t>(slots, std::max<uint64_t>(1, budget / per));
-    st = ensure_ws(g, static_cast<uint32_t>(slots), rs.mode, v.dk, rs.n_sources);
+    st = ensure_ws(g->ws, ctx->stream, g->n, g->m, static_cast<uint32_t>(slots), rs.mode, v.dk, rs.n_sources);
     if (st != NG_OK) return st;
     Workspace& w = g->ws;
     const hipStream_t s = ctx->stream;
@@ -461,6 +489,7 @@ ng_status run_batch(ng_graph* g, const RunSpec& rs, uint32_t* slots_used, double
     la.ws = &w;
     la.dist_out = d_dist_out;
     la.dist_out_kind = dok;
+    la.sorted = sorted;
     float ms = 0.0f;
     if (rs.n_sources) {
         NG_HIP(hipEventRecord(ctx->ev0, s));
@@ -788,6 +817,229 @@ ng_status ng_graph_last_stats(const ng_graph* g, ng_call_stats* out) {
     return NG_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+void cohort_free_device(ng_cohort* c) {
+    if (c->exec) (void)hipGraphExecDestroy(c->exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->exec = nullptr;
+    c->graph = nullptr;
+    dfree(c->d_gtab);
+    dfree(c->d_lend);
+    dfree(c->d_prefix);
+    dfree(c->d_chunk);
+    dfree(c->d_fbase);
+    dfree(c->d_seg_ptr);
+    dfree(c->d_seg_off);
+    dfree(c->d_flush);
+    dfree(c->d_bc);
+    c->built = false;
+}
+
+template <class T>
+ng_status upload(T** dst, const std::vector<T>& src, hipStream_t st) {
+    dfree(*dst);
+    NG_HIP(hipMalloc(reinterpret_cast<void**>(dst), sizeof(T) * std::max<size_t>(src.size(), 1)));
+    if (!src.empty()) NG_HIP(hipMemcpyAsync(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice, st));
+    return NG_OK;
+}
+
+// Sizes, partitions and captures the cohort launch:
+//   * items = (graph g, source s) in graph order; slot b gets the contiguous item range chunk[b]..chunk[b+1],
+//     cut so that every slot carries the same estimated cost (per-source cost weight of each graph);
+//   * each slot flushes its betweenness partials once per graph it touches, into its own flush region;
+//     the reduction sums a graph's regions in slot order, so results are deterministic;
+//   * the captured hipGraph holds fill kernels (no memset nodes), the batched kernel and the reduction.
+ng_status cohort_build(ng_cohort* c) {
+    ng_context* ctx = c->ctx;
+    const hipStream_t st = ctx->stream;
+    NG_HIP(hipSetDevice(ctx->device));
+    cohort_free_device(c);
+    const uint32_t G = static_cast<uint32_t>(c->graphs.size());
+    std::vector<CohortGraph> gtab(G);
+    std::vector<double> eff_delta(G);
+    c->prefix.assign(G + 1, 0);
+    c->n_max = 0;
+    c->m_max = 0;
+    for (uint32_t gi = 0; gi < G; ++gi) {
+        ng_graph* g = c->graphs[gi];
+        ng_status s0 = ensure_view(g, 0);
+        if (s0 != NG_OK) return s0;
+        const View& v = g->views[0];
+        if (v.has_zero) return fail(NG_EUNSUPPORTED, "cohort graph %u has zero-length arcs (BrandesVariant.h:30-31)", gi);
+        if (gi == 0) c->dk = v.dk;
+        if (v.dk != c->dk) return fail(NG_EUNSUPPORTED, "cohort graphs must share one distance type (graph %u differs)", gi);
+        const uint64_t np = static_cast<uint64_t>(c->prefix[gi]) + g->n;
+        if (np > 0xFFFFFFF0ull) return fail(NG_EINVAL, "cohort too large: more than 2^32 sources in total");
+        c->prefix[gi + 1] = static_cast<uint32_t>(np);
+        c->n_max = std::max(c->n_max, g->n);
+        c->m_max = std::max(c->m_max, g->m);
+        double d = pick_delta(g, v, 0, 1, c->delta[gi]);
+        if (c->dk == DKind::U32 || c->dk == DKind::U64) d = std::max(1.0, std::round(d));
+        eff_delta[gi] = d;
+        gtab[gi] = CohortGraph{v.off, v.tgt, v.len, nullptr, v.perm, g->n, 0u, d};
+    }
+    c->total = c->prefix[G];
+    // per-graph light-prefix ends for the cohort's own widths
+    NG_HIP(hipMalloc(&c->d_lend, 4 * std::max<size_t>(c->total, 1)));
+    for (uint32_t gi = 0; gi < G; ++gi) {
+        const View& v = c->graphs[gi]->views[0];
+        uint32_t* lend = c->d_lend + c->prefix[gi];
+        NG_HIP(launch_light_end(v.dk, c->graphs[gi]->n, v.off, v.len, eff_delta[gi], lend, st));
+        gtab[gi].lend = lend;
+    }
+    // slots
+    const bool sorted = settle_sorted(c->dk, MODE_BC, c->n_max);
+    const int bpc_max = batch_blocks_per_cu(c->dk, MODE_BC, c->n_max, sorted);
+    const int bpc = c->bpc > 0 ? std::min(c->bpc, bpc_max) : bpc_max;
+    uint64_t slots = static_cast<uint64_t>(bpc) * static_cast<uint64_t>(ctx->cus);
+    slots = std::min<uint64_t>(slots, std::max<uint32_t>(c->total, 1));
+    size_t free_b = 0, total_b = 0;
+    NG_HIP(hipMemGetInfo(&free_b, &total_b));
+    const uint64_t per = slot_bytes(c->n_max, c->m_max, MODE_BC, 8, log_capacity(c->n_max, c->m_max)) + 16ull * c->n_max;
+    const uint64_t held = static_cast<uint64_t>(c->ws.slots) * per;
+    const uint64_t budget = static_cast<uint64_t>(static_cast<double>(free_b + held) * 0.5);
+    slots = std::min<uint64_t>(slots, std::max<uint64_t>(1, budget / per));
+    c->slots = static_cast<uint32_t>(slots);
+    // cost-balanced contiguous partition of the items
+    std::vector<uint32_t> chunk(slots + 1, c->total);
+    {
+        double C = 0.0;
+        for (uint32_t gi = 0; gi < G; ++gi) C += static_cast<double>(c->graphs[gi]->n) * c->cost[gi];
+        chunk[0] = 0;
+        uint64_t b = 1;
+        double acc = 0.0;
+        for (uint32_t gi = 0; gi < G && b < slots; ++gi) {
+            for (uint32_t i = 0; i < c->graphs[gi]->n && b < slots; ++i) {
+                acc += c->cost[gi];
+                while (b < slots && acc >= C * static_cast<double>(b) / static_cast<double>(slots)) chunk[b++] = c->prefix[gi] + i + 1;
+            }
+        }
+        while (b < slots) chunk[b++] = c->total;
+        chunk[slots] = c->total;
+    }
+    // flush layout
+    auto graph_of = [&](uint32_t item) {
+        return static_cast<uint32_t>(std::upper_bound(c->prefix.begin(), c->prefix.end(), item) - c->prefix.begin() - 1);
+    };
+    std::vector<uint64_t> fbase(slots, 0);
+    std::vector<std::vector<uint64_t>> segs(G);
+    uint64_t F = 0;
+    for (uint64_t b = 0; b < slots; ++b) {
+        fbase[b] = F;
+        if (chunk[b] >= chunk[b + 1]) continue;
+        const uint32_t g0 = graph_of(chunk[b]), g1 = graph_of(chunk[b + 1] - 1);
+        for (uint32_t gi = g0; gi <= g1; ++gi) {
+            segs[gi].push_back(F);
+            F += c->graphs[gi]->n;
+        }
+    }
+    std::vector<uint32_t> seg_ptr(G + 1, 0);
+    std::vector<uint64_t> seg_off;
+    for (uint32_t gi = 0; gi < G; ++gi) {
+        seg_ptr[gi + 1] = seg_ptr[gi] + static_cast<uint32_t>(segs[gi].size());
+        seg_off.insert(seg_off.end(), segs[gi].begin(), segs[gi].end());
+    }
+    ng_status s1 = upload(&c->d_gtab, gtab, st);
+    if (s1 == NG_OK) s1 = upload(&c->d_prefix, c->prefix, st);
+    if (s1 == NG_OK) s1 = upload(&c->d_chunk, chunk, st);
+    if (s1 == NG_OK) s1 = upload(&c->d_fbase, fbase, st);
+    if (s1 == NG_OK) s1 = upload(&c->d_seg_ptr, seg_ptr, st);
+    if (s1 == NG_OK) s1 = upload(&c->d_seg_off, seg_off, st);
+    if (s1 != NG_OK) return s1;
+    NG_HIP(hipMalloc(&c->d_flush, 8 * std::max<uint64_t>(F, 1)));
+    NG_HIP(hipMalloc(&c->d_bc, 8 * std::max<uint32_t>(c->total, 1)));
+    ng_status s2 = ensure_ws(c->ws, st, c->n_max, c->m_max, c->slots, MODE_BC, c->dk, c->total);
+    if (s2 != NG_OK) return s2;
+    NG_HIP(hipStreamSynchronize(st));
+
+    LaunchArgs la{};
+    la.dk = c->dk;
+    la.mode = MODE_BC;
+    la.n = c->n_max;
+    la.view = nullptr;
+    la.sources = nullptr;
+    la.n_sources = c->total;
+    la.delta = 0.0;
+    la.grid = c->slots;
+    la.ws = &c->ws;
+    la.dist_out = nullptr;
+    la.dist_out_kind = 0;
+    la.sorted = sorted;
+    la.gtab = c->d_gtab;
+    la.n_graphs = G;
+    la.node_prefix = c->d_prefix;
+    la.chunk = c->d_chunk;
+    la.flush = c->d_flush;
+    la.flush_base = c->d_fbase;
+    Workspace& w = c->ws;
+    NG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    hipError_t e = launch_fill_f64(w.bc_slot, static_cast<uint64_t>(c->slots) * w.n, 0.0, st);
+    if (e == hipSuccess) e = launch_fill_u64(w.arcs_reached, 32, 0, st);
+    if (e == hipSuccess) e = launch_fill_u32(w.err, 1, 0u, st);
+    if (e == hipSuccess) e = launch_batch(la, st);
+    if (e == hipSuccess) e = launch_cohort_reduce(c->d_flush, c->d_prefix, G, c->d_seg_ptr, c->d_seg_off, c->total, c->d_bc, st);
+    hipGraph_t graph = nullptr;
+    const hipError_t ee = hipStreamEndCapture(st, &graph);
+    if (e != hipSuccess || ee != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return fail(NG_EDEVICE, "cohort graph capture failed: %s", hipGetErrorString(e != hipSuccess ? e : ee));
+    }
+    c->graph = graph;
+    NG_HIP(hipGraphInstantiate(&c->exec, c->graph, nullptr, nullptr, 0));
+    c->built = true;
+    return NG_OK;
+}
+
+// One replay of the captured cohort launch; fills c->last.
+ng_status cohort_replay(ng_cohort* c) {
+    ng_context* ctx = c->ctx;
+    const hipStream_t st = ctx->stream;
+    NG_HIP(hipSetDevice(ctx->device));
+    if (!c->built) {
+        ng_status s0 = cohort_build(c);
+        if (s0 != NG_OK) return s0;
+    }
+    NG_HIP(hipEventRecord(ctx->ev0, st));
+    NG_HIP(hipGraphLaunch(c->exec, st));
+    NG_HIP(hipEventRecord(ctx->ev1, st));
+    const hipError_t se = hipStreamSynchronize(st);
+    if (se != hipSuccess) return fail(NG_EDEVICE, "cohort launch failed: %s", hipGetErrorString(se));
+    float ms = 0.0f;
+    NG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    uint32_t err = 0;
+    unsigned long long ctrs[32] = {};
+    NG_HIP(hipMemcpy(&err, c->ws.err, 4, hipMemcpyDeviceToHost));
+    NG_HIP(hipMemcpy(ctrs, c->ws.arcs_reached, 8 * 32, hipMemcpyDeviceToHost));
+    if (err & ERR_KAHN) return fail(NG_EDEVICE, "tight-arc DAG has a cycle (zero-length arcs?)");
+    if (err & ERR_GUARD) return fail(NG_EDEVICE, "iteration guard hit in the batched kernel");
+    if (err & ERR_OVERFLOW) return fail(NG_EDEVICE, "queue overflow in the batched kernel");
+    ng_call_stats& l = c->last;
+    l = ng_call_stats{};
+    l.kernel_ms = ms;
+    l.launches = 1;
+    l.slots = c->slots;
+    l.delta = 0.0;
+    l.arcs_reached = ctrs[0];
+    l.flags = err;
+    l.phases = ctrs[1];
+    l.buckets = ctrs[2];
+    l.light_arcs = ctrs[3];
+    l.settle_arcs = ctrs[4];
+    l.logged_arcs = ctrs[5];
+    l.layers = ctrs[6];
+    l.probes = ctrs[7];
+    l.improvements = ctrs[8];
+    for (int sct = 0; sct < 8; ++sct) l.section_ticks[sct] = ctrs[9 + sct];
+    return NG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 ng_status ng_cohort_create(ng_context* ctx, ng_graph* const* graphs, uint32_t n_graphs, ng_cohort** out) {
     if (!out) return fail(NG_EINVAL, "null output pointer");
     *out = nullptr;
@@ -797,19 +1049,85 @@ ng_status ng_cohort_create(ng_context* ctx, ng_graph* const* graphs, uint32_t n_
     for (uint32_t i = 0; i < n_graphs; ++i) {
         if (!graphs[i] || graphs[i]->ctx != ctx) return fail(NG_EINVAL, "graph %u is null or belongs to another context", i);
         c->graphs.push_back(graphs[i]);
+        c->delta.push_back(0.0);
+        // default cost model of one source: a pass over the arcs and the nodes it reaches
+        c->cost.push_back(static_cast<double>(graphs[i]->m) + 2.0 * graphs[i]->n + 64.0);
     }
     *out = c.release();
     return NG_OK;
 }
 
-void ng_cohort_destroy(ng_cohort* c) { delete c; }
+void ng_cohort_destroy(ng_cohort* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->ctx->device);
+    (void)hipStreamSynchronize(c->ctx->stream);
+    cohort_free_device(c);
+    free_ws(c->ws);
+    delete c;
+}
 
 ng_status ng_cohort_brandes(ng_cohort* c, double* bc_out, ng_sssp_summary* summaries_out, double* kernel_ms) {
-    (void)c;
-    (void)bc_out;
-    (void)summaries_out;
-    (void)kernel_ms;
-    return fail(NG_EUNSUPPORTED, "cohort launch not built yet");
+    if (!c) return fail(NG_EINVAL, "null cohort");
+    if (!bc_out) return fail(NG_EINVAL, "null betweenness buffer");
+    ng_status st = cohort_replay(c);
+    if (st != NG_OK) return st;
+    NG_HIP(hipMemcpy(bc_out, c->d_bc, 8 * static_cast<size_t>(c->total), hipMemcpyDeviceToHost));
+    if (summaries_out && c->total) {
+        NG_HIP(hipMemcpy(summaries_out, c->ws.summaries, sizeof(ng_sssp_summary) * c->total, hipMemcpyDeviceToHost));
+    }
+    if (kernel_ms) *kernel_ms = c->last.kernel_ms;
+    return NG_OK;
+}
+
+ng_status ng_cohort_autotune(ng_cohort* c, uint32_t n_sample) {
+    if (!c) return fail(NG_EINVAL, "null cohort");
+    if (n_sample == 0) return fail(NG_EINVAL, "empty sample");
+    // per graph: the bucket width, then the slot-time of one source at that width (partition weight)
+    for (size_t gi = 0; gi < c->graphs.size(); ++gi) {
+        ng_graph* g = c->graphs[gi];
+        const uint32_t k = std::min(n_sample, g->n);
+        std::vector<uint32_t> sample(k);
+        for (uint32_t i = 0; i < k; ++i) sample[i] = static_cast<uint32_t>((static_cast<uint64_t>(i) * g->n) / k);
+        double d = 0.0;
+        ng_status st = ng_autotune_delta(g, NG_LEN_ABS, 1, sample.data(), k, &d);
+        if (st != NG_OK) return st;
+        c->delta[gi] = d;
+        RunSpec rs{0, MODE_BC, d, sample.data(), k, nullptr};
+        uint32_t used = 0;
+        st = run_batch(g, rs, &used, nullptr);
+        if (st != NG_OK) return st;
+        const double busy = static_cast<double>(std::min<uint32_t>(used, k));
+        c->cost[gi] = std::max(1e-6, g->last.kernel_ms * busy / static_cast<double>(k));
+    }
+    // resident workgroups per CU: the occupancy maximum and two lower values, one whole replay each
+    c->bpc = 0;
+    c->built = false;
+    ng_status st = cohort_replay(c);
+    if (st != NG_OK) return st;
+    const uint32_t max_slots = c->slots;
+    const int bpc_max = std::max(1, static_cast<int>(max_slots / std::max(1, c->ctx->cus)));
+    int best = 0;
+    double best_ms = c->last.kernel_ms;
+    for (int b = bpc_max - 1; b >= std::max(1, bpc_max - 2); --b) {
+        c->bpc = b;
+        c->built = false;
+        st = cohort_replay(c);
+        if (st != NG_OK) return st;
+        if (c->last.kernel_ms < best_ms) {
+            best_ms = c->last.kernel_ms;
+            best = b;
+        }
+    }
+    c->bpc = best;
+    c->built = false;
+    return NG_OK;
+}
+
+ng_status ng_cohort_last_stats(const ng_cohort* c, ng_call_stats* out) {
+    if (!c) return fail(NG_EINVAL, "null cohort");
+    if (!out) return fail(NG_EINVAL, "null output pointer");
+    *out = c->last;
+    return NG_OK;
 }
 
 }  // extern "C"

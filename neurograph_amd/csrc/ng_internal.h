@@ -69,6 +69,8 @@ struct Workspace {
     double* bc_out = nullptr;   // [n]
     double* grad1 = nullptr;    // [m]
     double* grad2 = nullptr;    // [m]
+    DKind dist_kind = DKind::U32;  // element kind the dist array currently holds +infinity in
+    bool dist_valid = false;
 };
 
 // Device error bits reported by the batched kernel.
@@ -78,6 +80,18 @@ enum : uint32_t {
     ERR_BUCKET = 4u,        // informational: a settle pass produced in-bucket work
     ERR_GUARD = 8u,         // iteration guard hit: results of that source are incomplete
     ERR_OVERFLOW = 16u      // a queue outgrew its capacity: results of that source are incomplete
+};
+
+// One member graph of a cohort launch (device pointers into the graph's |w| view).
+struct CohortGraph {
+    const uint32_t* off;
+    const uint32_t* tgt;
+    const void* len;
+    const uint32_t* lend;
+    const uint32_t* perm;
+    uint32_t n;
+    uint32_t pad;
+    double delta;  // bucket width (integral for integer lengths)
 };
 
 struct LaunchArgs {
@@ -92,6 +106,14 @@ struct LaunchArgs {
     Workspace* ws;
     void* dist_out;           // device, optional
     int dist_out_kind;        // ng_dist_kind
+    bool sorted;              // settle passes expand each bucket's settled nodes in increasing id
+    // cohort launch (gtab != nullptr; view, sources and n_sources unused; n = the largest member's n)
+    const CohortGraph* gtab = nullptr;
+    uint32_t n_graphs = 0;
+    const uint32_t* node_prefix = nullptr;  // [n_graphs+1] item prefix (= node prefix) per graph
+    const uint32_t* chunk = nullptr;        // [grid+1] item range per slot
+    double* flush = nullptr;
+    const uint64_t* flush_base = nullptr;   // [grid] start of each slot's flush region
 };
 
 // Launch wrappers (ng_kernels.hip). All enqueue on `stream`.
@@ -99,12 +121,16 @@ hipError_t launch_batch(const LaunchArgs& a, hipStream_t stream);
 hipError_t launch_light_end(DKind dk, uint32_t n, const uint32_t* off, const void* len, double delta, uint32_t* lend,
                             hipStream_t stream);
 hipError_t launch_reduce_slots(const double* bc_slot, uint32_t slots, uint32_t n, double* out, hipStream_t stream);
+// out[node_prefix[g] + v] = sum over k in [seg_ptr[g], seg_ptr[g+1]) of flush[seg_off[k] + v] (slot order).
+hipError_t launch_cohort_reduce(const double* flush, const uint32_t* node_prefix, uint32_t n_graphs, const uint32_t* seg_ptr,
+                                const uint64_t* seg_off, uint32_t total_nodes, double* out, hipStream_t stream);
+hipError_t launch_fill_u64(uint64_t* p, uint64_t count, uint64_t value, hipStream_t stream);
 hipError_t launch_fill_u32(uint32_t* p, uint64_t count, uint32_t value, hipStream_t stream);
 hipError_t launch_fill_f64(double* p, uint64_t count, double value, hipStream_t stream);
 hipError_t launch_fill_dist(DKind dk, void* p, uint64_t count, hipStream_t stream);  // fills with +infinity / max
 hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t stream);  // bytes % 16 == 0
 // Resident workgroups per CU of the batched kernel for (dk, mode).
-int batch_blocks_per_cu(DKind dk, int mode, uint32_t n);
+int batch_blocks_per_cu(DKind dk, int mode, uint32_t n, bool sorted);
 constexpr int kBlock = 256;
 
 }  // namespace ng

@@ -365,7 +365,16 @@ struct KArgs {
     uint64_t words;
     uint2* log;
     uint32_t logcap;
+    uint32_t stride;    // nodes per slot in the workspace layout (>= n)
+    // cohort launch (gtab != nullptr): graph table, item prefix per graph, item range per slot, flush layout
+    const CohortGraph* gtab;
+    uint32_t n_graphs;
+    const uint32_t* node_prefix;
+    const uint32_t* chunk;
+    double* flush;
+    const uint64_t* flush_base;
     uint32_t bm_words;  // words of the LDS bitmap of finalized nodes (0: no bitmap)
+    uint32_t sorted;    // 1: a second LDS bitmap orders each bucket's settled nodes by id
     double* bc_slot;
     double* grad1;
     double* grad2;
@@ -429,19 +438,34 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
     // below hi while the candidate is at least hi, so the arc can neither improve nor be tight.
     extern __shared__ uint32_t bm[];
     const bool use_bm = a.bm_words != 0;
+    // With `sorted`, nodes registered in the current bucket are also marked in bn; before the settle pass
+    // the bucket's part of the settled list is written by enumerating bn, i.e. in increasing node id, so
+    // that consecutive tiles of the settle pass expand nodes with nearby ids (and, for a locality-ordered
+    // numbering, nearby targets).
+    const bool sorted = use_bm && a.sorted != 0;
+    uint32_t* const bn = bm + a.bm_words;
 
     const uint32_t tid = threadIdx.x;
     const uint32_t slot = blockIdx.x;
-    const uint32_t n = a.n;
-    const size_t so = static_cast<size_t>(slot) * n;
+    // slot layout uses the workspace stride; the graph's own size (n) and arrays can change per source in a
+    // cohort launch
+    const uint32_t stride = a.stride;
+    const size_t so = static_cast<size_t>(slot) * stride;
+    uint32_t n = a.n;
+    const uint32_t* off = a.off;
+    const uint32_t* tgt = a.tgt;
+    const L* len = a.len;
+    const uint32_t* lend = a.lend;
+    const uint32_t* perm = a.perm;
+    uint32_t bmw = a.bm_words;  // bitmap words of the current graph
     B* dist = a.dist + so;
     uint32_t* fstamp = a.fstamp + so;
     uint32_t* sstamp = a.sstamp + so;
     uint32_t* pstamp = a.pstamp + so;
     uint32_t* lists = a.lists + so * 3;
     uint32_t* const Fa = lists;
-    uint32_t* const Fb = lists + n;
-    uint32_t* const Sl = lists + 2 * static_cast<size_t>(n);
+    uint32_t* const Fb = lists + stride;
+    uint32_t* const Sl = lists + 2 * static_cast<size_t>(stride);
     const uint32_t pcap = a.pcap;
     PileEntry<B>* const Pa = static_cast<PileEntry<B>*>(a.pile) + static_cast<size_t>(slot) * 2 * pcap;
     PileEntry<B>* const Pb = Pa + pcap;
@@ -459,7 +483,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
         delta1 = a.delta1 + so;
         indeg = a.indeg + so;
         order = a.order + so;
-        layers = a.layers + static_cast<size_t>(slot) * (n + 1);
+        layers = a.layers + static_cast<size_t>(slot) * (stride + 1);
         bits = a.bits + static_cast<size_t>(slot) * a.words;
         log = a.log + static_cast<size_t>(slot) * a.logcap;
         if constexpr ((MODE & MODE_ARC2) != 0) delta2 = a.delta2 + so;
@@ -467,14 +491,58 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
     }
     uint32_t* ctr = a.counters + static_cast<size_t>(slot) * 4;
     uint32_t phase = ctr[0], gen = ctr[1], pgen = ctr[2];
-    const L delta = a.delta;
+    L delta = a.delta;
     // work counters (uniform per workgroup, except the arc counts which lane 0 accumulates from the tiles)
     unsigned long long c_phases = 0, c_buckets = 0, c_light = 0, c_settle = 0, c_logged = 0, c_layers = 0;
     unsigned long long c_probe = 0, c_improve = 0;
 
-    for (uint32_t si = blockIdx.x; si < a.n_sources; si += gridDim.x) {
-        if (phase > 0xE0000000u || gen > 0xE0000000u || pgen > 0xE0000000u) {
+    // Plain launch: sources blockIdx.x, +gridDim.x, ... of a.sources. Cohort launch (a.gtab != nullptr): the
+    // contiguous item range [chunk[slot], chunk[slot+1]) of the cohort's (graph, source) items in graph order;
+    // when the graph changes, the slot's betweenness partials of the previous graph are flushed to the slot's
+    // flush region (and cleared), which a reduction sums in slot order.
+    const bool cohort = a.gtab != nullptr;
+    uint32_t cur_g = 0xFFFFFFFFu;
+    size_t fpos = cohort ? a.flush_base[slot] : 0;
+    auto flush_bc = [&]() {
+        if constexpr ((MODE & MODE_BC) != 0) {
             for (uint32_t v = tid; v < n; v += kBlock) {
+                a.flush[fpos + v] = bc_slot[v];
+                bc_slot[v] = 0.0;
+            }
+            fpos += n;
+        }
+    };
+    const uint32_t it_first = cohort ? a.chunk[slot] : blockIdx.x;
+    const uint32_t it_end = cohort ? a.chunk[slot + 1] : a.n_sources;
+    const uint32_t it_step = cohort ? 1u : gridDim.x;
+    for (uint32_t si = it_first; si < it_end; si += it_step) {
+        uint32_t s;
+        if (cohort) {
+            uint32_t lo = 0, hi = a.n_graphs;  // graph g holds items [node_prefix[g], node_prefix[g+1])
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (a.node_prefix[mid] <= si) lo = mid; else hi = mid;
+            }
+            if (lo != cur_g) {
+                if (cur_g != 0xFFFFFFFFu) flush_bc();
+                const CohortGraph& cg = a.gtab[lo];
+                n = cg.n;
+                off = cg.off;
+                tgt = cg.tgt;
+                len = static_cast<const L*>(cg.len);
+                lend = cg.lend;
+                perm = cg.perm;
+                delta = static_cast<L>(cg.delta);
+                bmw = use_bm ? (n + 31) / 32 : 0u;
+                cur_g = lo;
+                bsync();
+            }
+            s = si - a.node_prefix[lo];
+        } else {
+            s = a.sources[si];
+        }
+        if (phase > 0xE0000000u || gen > 0xE0000000u || pgen > 0xE0000000u) {
+            for (uint32_t v = tid; v < stride; v += kBlock) {
                 stc(&fstamp[v], 0u);
                 stc(&sstamp[v], 0u);
                 stc(&pstamp[v], 0u);
@@ -485,8 +553,9 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
         ++gen;
         ++phase;
         if (use_bm)
-            for (uint32_t w = tid; w < a.bm_words; w += kBlock) bm[w] = 0u;
-        const uint32_t s = a.sources[si];
+            for (uint32_t w = tid; w < bmw; w += kBlock) bm[w] = 0u;
+        if (sorted)
+            for (uint32_t w = tid; w < bmw; w += kBlock) bn[w] = 0u;
         if (tid == 0) {
 #pragma unroll
             for (int k = 0; k < 8; ++k) S.cnt[k] = 0;
@@ -580,11 +649,11 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                 ++c_phases;
                 NG_TICK(t_inner);
                 expand_relax<NG_UNROLL>(
-                    Fc, nF, S, a.tgt, a.len,
+                    Fc, nF, S, tgt, len,
                     [&](uint32_t v, B& dv, uint32_t& beg, uint32_t& cnt) {
                         dv = ldc(&dist[v]);
-                        beg = a.off[v];
-                        cnt = a.lend[v] - beg;
+                        beg = off[v];
+                        cnt = lend[v] - beg;
                         c_light += cnt;
                     },
                     probe,
@@ -601,11 +670,13 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                         if (use_bm) {
                             const uint32_t bit = 1u << (v & 31);
                             fresh = (atomicOr(&bm[v >> 5], bit) & bit) == 0u;
+                            if (fresh && sorted) atomicOr(&bn[v >> 5], bit);
                         } else {
                             fresh = wg_exch(&sstamp[v], gen) != gen;
                         }
                     }
-                    push(fresh, v, Sl, &S.cnt[2], n);
+                    // sorted: only counted here; the list is written from bn before the settle pass
+                    push(fresh, v, Sl, &S.cnt[2], sorted ? 0u : n);
                 }
                 bsync();
                 nF = clampn(S.cnt[0]);
@@ -622,15 +693,37 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
             // ---- settle pass: every settled node's row once, with its final distance
             const uint32_t nS = clampn(S.cnt[2]);
             if (aborted) break;
+            if (sorted && nS > settle_from) {
+                uint32_t pos = settle_from;
+                for (uint32_t wb = 0; wb < bmw; wb += kBlock) {
+                    const uint32_t wi = wb + tid;
+                    uint32_t w = wi < bmw ? bn[wi] : 0u;
+                    uint32_t tot;
+                    const uint32_t ex = block_excl_scan(static_cast<uint32_t>(__popc(w)), &tot, S.wave);
+                    if (w != 0u) {
+                        bn[wi] = 0u;
+                        uint32_t k = pos + ex;
+                        while (w) {
+                            const uint32_t b = static_cast<uint32_t>(__ffs(w) - 1);
+                            w &= w - 1;
+                            if (k < n) Sl[k] = (wi << 5) + b;
+                            ++k;
+                        }
+                    }
+                    pos += tot;
+                    if (pos >= nS) break;  // uniform: tot is the block-wide count
+                }
+                bsync();
+            }
             if (nS > settle_from) {
                 NG_TICK(t_settle);
                 const uint32_t nph = phase + 1;
                 expand_relax<NG_UNROLL>(
-                    Sl + settle_from, nS - settle_from, S, a.tgt, a.len,
+                    Sl + settle_from, nS - settle_from, S, tgt, len,
                     [&](uint32_t v, B& dv, uint32_t& beg, uint32_t& cnt) {
                         dv = ldc(&dist[v]);
-                        beg = (MODE != MODE_SSSP) ? a.off[v] : a.lend[v];
-                        cnt = a.off[v + 1] - beg;
+                        beg = (MODE != MODE_SSSP) ? off[v] : lend[v];
+                        cnt = off[v + 1] - beg;
                         c_settle += cnt;
                     },
                     probe,
@@ -736,7 +829,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                 else sd += T::val(d);
                 if (d != B(0)) se += 1.0 / T::val(d);
                 mx = (static_cast<unsigned long long>(d) > mx) ? static_cast<unsigned long long>(d) : mx;
-                arcs += a.off[v + 1] - a.off[v];
+                arcs += off[v + 1] - off[v];
             }
             auto addu = [](unsigned long long x, unsigned long long y) { return x + y; };
             auto maxu = [](unsigned long long x, unsigned long long y) { return x > y ? x : y; };
@@ -791,8 +884,8 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
             const uint32_t nlog = S.cnt[3];
             c_logged += nlog;
             auto mark = [&](uint32_t u, uint32_t e) {
-                const uint32_t t = a.tgt[e];
-                const B c = T::add(ldc(&dist[u]), a.len[e]);
+                const uint32_t t = tgt[e];
+                const B c = T::add(ldc(&dist[u]), len[e]);
                 if (c == ldc(&dist[t])) {
                     wg_or(&bits[e >> 5], 1u << (e & 31));
                     wg_add(&indeg[t], 1u);
@@ -809,8 +902,8 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                     Sl, nS, S,
                     [&](uint32_t v, B& dv, uint32_t& beg, uint32_t& cnt) {
                         dv = 0;
-                        beg = a.off[v];
-                        cnt = a.off[v + 1] - beg;
+                        beg = off[v];
+                        cnt = off[v + 1] - beg;
                     },
                     [&](uint32_t v, B, uint32_t e) { mark(v, e); });
             }
@@ -831,14 +924,14 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                 for (uint32_t i = lb + tid; i < le; i += kBlock) {
                     const uint32_t u = order[i];
                     const double su = ldc(&sigma[u]);
-                    const uint32_t b = a.off[u], e_end = a.off[u + 1];
+                    const uint32_t b = off[u], e_end = off[u + 1];
                     if (b < e_end) {
                         for (uint32_t wi = b >> 5; wi <= (e_end - 1) >> 5; ++wi) {
                             uint32_t w = ldc(&bits[wi]) & range_mask(wi, b, e_end);
                             while (w) {
                                 const uint32_t e = (wi << 5) + static_cast<uint32_t>(__ffs(w) - 1);
                                 w &= w - 1;
-                                const uint32_t t = a.tgt[e];
+                                const uint32_t t = tgt[e];
                                 wg_add(&sigma[t], su);
                                 if (wg_sub(&indeg[t], 1u) == 1u) {
                                     const uint32_t pos = atomicAdd(&S.cnt[4], 1u);
@@ -860,7 +953,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                 for (uint32_t i = tid; i < nS; i += kBlock) {
                     const uint32_t u = Sl[i];
                     stc(&indeg[u], 0u);
-                    const uint32_t b = a.off[u], e_end = a.off[u + 1];
+                    const uint32_t b = off[u], e_end = off[u + 1];
                     if (b < e_end)
                         for (uint32_t wi = b >> 5; wi <= (e_end - 1) >> 5; ++wi) wg_and(&bits[wi], ~range_mask(wi, b, e_end));
                 }
@@ -875,7 +968,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                     const uint32_t u = order[i];
                     const double su = ldc(&sigma[u]);
                     double acc1 = 0.0, acc2 = 0.0;
-                    const uint32_t b = a.off[u], e_end = a.off[u + 1];
+                    const uint32_t b = off[u], e_end = off[u + 1];
                     if (b < e_end) {
                         for (uint32_t wi = b >> 5; wi <= (e_end - 1) >> 5; ++wi) {
                             uint32_t w = ldc(&bits[wi]) & range_mask(wi, b, e_end);
@@ -884,16 +977,16 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                             while (w) {
                                 const uint32_t e = (wi << 5) + static_cast<uint32_t>(__ffs(w) - 1);
                                 w &= w - 1;
-                                const uint32_t t = a.tgt[e];
+                                const uint32_t t = tgt[e];
                                 const double st = ldc(&sigma[t]);
                                 const double term1 = su * ((1.0 + ldc(&delta1[t])) / st);
                                 acc1 += term1;
-                                if constexpr ((MODE & MODE_ARC1) != 0) unsafeAtomicAdd(&a.grad1[a.perm[e]], term1);
+                                if constexpr ((MODE & MODE_ARC1) != 0) unsafeAtomicAdd(&a.grad1[perm[e]], term1);
                                 if constexpr ((MODE & MODE_ARC2) != 0) {
                                     const double dt = T::val(ldc(&dist[t]));
                                     const double term2 = su * ((1.0 / (dt * dt) + ldc(&delta2[t])) / st);
                                     acc2 += term2;
-                                    unsafeAtomicAdd(&a.grad2[a.perm[e]], term2);
+                                    unsafeAtomicAdd(&a.grad2[perm[e]], term2);
                                 }
                             }
                         }
@@ -932,6 +1025,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
         bsync();
         NG_TOCK(t_reset, 7);
     }
+    if (cohort && cur_g != 0xFFFFFFFFu) flush_bc();
     if (tid == 0) {
         ctr[0] = phase;
         ctr[1] = gen;
@@ -992,10 +1086,11 @@ hipError_t fill(T* p, uint64_t count, T value, hipStream_t stream) {
     return hipGetLastError();
 }
 
-// Words of the finalized-node bitmap kept in LDS, or 0 when it would not fit next to the static LDS.
-uint32_t bitmap_words(uint32_t n) {
+// Words of the finalized-node bitmap kept in LDS, or 0 when it (and, with `sorted`, its twin) would not fit
+// next to the static LDS.
+uint32_t bitmap_words(uint32_t n, bool sorted) {
     const uint32_t w = (n + 31) / 32;
-    return w * 4 <= 96 * 1024 ? w : 0u;
+    return w * 4 * (sorted ? 2 : 1) <= 96 * 1024 ? w : 0u;
 }
 
 template <class D, int MODE>
@@ -1004,14 +1099,24 @@ hipError_t launch_typed(const LaunchArgs& la, hipStream_t stream) {
     using B = typename T::B;
     using L = typename T::L;
     KArgs<D> k{};
-    const View& vw = *la.view;
     Workspace& ws = *la.ws;
     k.n = la.n;
-    k.off = vw.off;
-    k.tgt = vw.tgt;
-    k.len = static_cast<const L*>(vw.len);
-    k.lend = vw.lend;
-    k.perm = vw.perm;
+    k.stride = ws.n;
+    if (la.gtab == nullptr) {
+        const View& vw = *la.view;
+        k.off = vw.off;
+        k.tgt = vw.tgt;
+        k.len = static_cast<const L*>(vw.len);
+        k.lend = vw.lend;
+        k.perm = vw.perm;
+    } else {
+        k.gtab = la.gtab;
+        k.n_graphs = la.n_graphs;
+        k.node_prefix = la.node_prefix;
+        k.chunk = la.chunk;
+        k.flush = la.flush;
+        k.flush_base = la.flush_base;
+    }
     k.sources = la.sources;
     k.n_sources = la.n_sources;
     if constexpr (T::kInt) {
@@ -1046,8 +1151,9 @@ hipError_t launch_typed(const LaunchArgs& la, hipStream_t stream) {
     k.err = ws.err;
     k.dist_out = la.dist_out;
     k.dist_out_kind = la.dist_out_kind;
-    k.bm_words = bitmap_words(la.n);
-    hipLaunchKernelGGL((batch_kernel<D, MODE>), dim3(la.grid), dim3(kBlock), k.bm_words * 4, stream, k);
+    k.bm_words = bitmap_words(la.n, la.sorted);
+    k.sorted = (la.sorted && k.bm_words) ? 1u : 0u;
+    hipLaunchKernelGGL((batch_kernel<D, MODE>), dim3(la.grid), dim3(kBlock), k.bm_words * 4 * (k.sorted ? 2 : 1), stream, k);
     return hipGetLastError();
 }
 
@@ -1063,10 +1169,11 @@ hipError_t launch_mode(const LaunchArgs& a, hipStream_t stream) {
 }
 
 template <class D>
-int blocks_per_cu_mode(int mode, uint32_t n) {
+int blocks_per_cu_mode(int mode, uint32_t n, bool sorted) {
     int nb = 0;
     hipError_t e = hipErrorInvalidValue;
-    const size_t dyn = static_cast<size_t>(bitmap_words(n)) * 4;
+    const uint32_t bw = bitmap_words(n, sorted);
+    const size_t dyn = static_cast<size_t>(bw) * 4 * ((sorted && bw) ? 2 : 1);
     switch (mode) {
     case MODE_SSSP: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<D, MODE_SSSP>, kBlock, dyn); break;
     case MODE_BC: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<D, MODE_BC>, kBlock, dyn); break;
@@ -1090,12 +1197,12 @@ hipError_t launch_batch(const LaunchArgs& a, hipStream_t stream) {
     return hipErrorInvalidValue;
 }
 
-int batch_blocks_per_cu(DKind dk, int mode, uint32_t n) {
+int batch_blocks_per_cu(DKind dk, int mode, uint32_t n, bool sorted) {
     switch (dk) {
-    case DKind::U32: return blocks_per_cu_mode<uint32_t>(mode, n);
-    case DKind::U64: return blocks_per_cu_mode<uint64_t>(mode, n);
-    case DKind::F32: return blocks_per_cu_mode<float>(mode, n);
-    case DKind::F64: return blocks_per_cu_mode<double>(mode, n);
+    case DKind::U32: return blocks_per_cu_mode<uint32_t>(mode, n, sorted);
+    case DKind::U64: return blocks_per_cu_mode<uint64_t>(mode, n, sorted);
+    case DKind::F32: return blocks_per_cu_mode<float>(mode, n, sorted);
+    case DKind::F64: return blocks_per_cu_mode<double>(mode, n, sorted);
     }
     return 1;
 }
@@ -1128,6 +1235,31 @@ hipError_t launch_light_end(DKind dk, uint32_t n, const uint32_t* off, const voi
     return hipGetLastError();
 }
 
+namespace {
+__global__ void cohort_reduce_kernel(const double* flush, const uint32_t* node_prefix, uint32_t n_graphs, const uint32_t* seg_ptr,
+                                     const uint64_t* seg_off, uint32_t total, double* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    uint32_t lo = 0, hi = n_graphs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (node_prefix[mid] <= i) lo = mid; else hi = mid;
+    }
+    const uint32_t v = i - node_prefix[lo];
+    double acc = 0.0;
+    for (uint32_t k = seg_ptr[lo]; k < seg_ptr[lo + 1]; ++k) acc += flush[seg_off[k] + v];
+    out[i] = acc;
+}
+}  // namespace
+
+hipError_t launch_cohort_reduce(const double* flush, const uint32_t* node_prefix, uint32_t n_graphs, const uint32_t* seg_ptr,
+                                const uint64_t* seg_off, uint32_t total_nodes, double* out, hipStream_t stream) {
+    if (total_nodes == 0) return hipSuccess;
+    hipLaunchKernelGGL(cohort_reduce_kernel, dim3((total_nodes + 255) / 256), dim3(256), 0, stream, flush, node_prefix, n_graphs,
+                       seg_ptr, seg_off, total_nodes, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_reduce_slots(const double* bc_slot, uint32_t slots, uint32_t n, double* out, hipStream_t stream) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(reduce_slots_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, bc_slot, slots, n, out);
@@ -1149,6 +1281,9 @@ hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s
 
 hipError_t launch_fill_u32(uint32_t* p, uint64_t count, uint32_t value, hipStream_t stream) {
     return fill<uint32_t>(p, count, value, stream);
+}
+hipError_t launch_fill_u64(uint64_t* p, uint64_t count, uint64_t value, hipStream_t stream) {
+    return fill<unsigned long long>(reinterpret_cast<unsigned long long*>(p), count, static_cast<unsigned long long>(value), stream);
 }
 hipError_t launch_fill_f64(double* p, uint64_t count, double value, hipStream_t stream) {
     return fill<double>(p, count, value, stream);

@@ -244,3 +244,20 @@ def test_small_pile_forces_exact_bucket_advance(ctx, monkeypatch):
             np.testing.assert_array_equal(dist, ref)
         bc = g.brandes(srcs)
         np.testing.assert_allclose(bc, orc.brandes(c, sources=srcs)["bc"], rtol=1e-9, atol=1e-9)
+
+
+@pytest.mark.parametrize("order", ["0", "1"])
+def test_settle_order_variants(ctx, monkeypatch, order):
+    """Both settle-pass orders (discovery order, and increasing id through the second LDS bitmap) give the
+    oracle's distances bit-exactly and its betweenness within 1e-9."""
+    monkeypatch.setenv("NG_SORTED_SETTLE", order)
+    for wdtype, seed in ((np.int32, 5), (np.float64, 6)):
+        c = rand_graph(4000, 40000, seed, wdtype=wdtype, wlo=1, whi=9)
+        g = dev_graph(ctx, c)
+        srcs = np.arange(0, 4000, 29)
+        ref = np.stack([orc.sssp(c, int(s))[0] for s in srcs])
+        for delta in (0.0, 0.7, 5.0):
+            _, dist = g.sssp(srcs, delta=delta, want_dist=True)
+            np.testing.assert_array_equal(dist, ref)
+        bc, summ = g.brandes(srcs, summaries=True)
+        np.testing.assert_allclose(bc, orc.brandes(c, sources=srcs)["bc"], rtol=1e-9, atol=1e-9)
