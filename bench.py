@@ -12,6 +12,9 @@ Workloads (default spatial50k):
   spatial50k   50k nodes uniform in a 1000^3 um box, p ~ exp(-d/150 um), mean out-degree 20, FP64 lognormal(0,0.5),
                all 50k sources
   spatial200k  200k nodes, mean out-degree 100 (~20M arcs), FP32 lognormal(0,0.5), 8192 sampled sources, batches of 1024
+  cohort64     64 spatial graphs, nodes U[1k, 20k], mean out-degree U[10, 60], FP32 lognormal(0,0.5), all sources of all
+               graphs; one step = one replay of the hipGraph-captured persistent cohort launch, reported beside the
+               same work as 64 plain launches (one ng_brandes_accumulate per graph)
 
 Roofline: algorithmic bytes per source B_bc = 2*(A*(s_i+s_w) + (V+1)*s_o) + V*(2*s_d + 32) (one pass over the
 CSR for the forward sweep, one for the backward sweep, per-node distance/sigma/delta/betweenness state) divided
@@ -61,6 +64,117 @@ def bytes_per_source(g) -> float:
     return 2.0 * (g.m * (4 + s_w) + (g.n + 1) * 4) + g.n * (2 * s_d + 32)
 
 
+def make_cohort(seed: int = 0):
+    """64 member graphs: n ~ U[1000, 20000], mean out-degree ~ U[10, 60] (seeded), spatial generator, FP32."""
+    import neurograph_amd as ng
+
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(1000, 20001, size=64)
+    ds = rng.uniform(10.0, 60.0, size=64)
+    return [ng.generate_spatial(int(n), float(d), box=1000.0, lam=150.0, wdist="lognormal", a=0.0, b=0.5,
+                                weights=np.float32, seed=1000 + i) for i, (n, d) in enumerate(zip(ns, ds))]
+
+
+def run_cohort(args):
+    import neurograph_amd as ng
+
+    t0 = time.perf_counter()
+    hgs = make_cohort()
+    gen_s = time.perf_counter() - t0
+    ctx = ng.Context(0)
+    gs = [ng.Graph(ctx, h) for h in hgs]
+    n_total = sum(h.n for h in hgs)
+    m_total = sum(h.m for h in hgs)
+    print(f"[bench] cohort64: {n_total} sources, {m_total} arcs in 64 graphs, generated in {gen_s:.1f}s",
+          file=sys.stderr, flush=True)
+    co = ng.Cohort(ctx, gs)
+    t0 = time.perf_counter()
+    co.autotune(256)
+    print(f"[bench] cohort autotune {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    steps = args.steps or 3
+    for _ in range(args.warmup):
+        co.brandes()
+    kernel_ms, arcs = [], 0
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _, _, ms = co.brandes()
+        kernel_ms.append(ms)
+        arcs += co.last_stats()["arcs_reached"]
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    # the same work as 64 plain launches (each graph's own tuned width, all its sources, one call per graph)
+    for g in gs:
+        g.brandes(delta=0.0)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    plain_kernel = 0.0
+    for _ in range(steps):
+        for g in gs:
+            g.brandes(delta=0.0)
+            plain_kernel += g.last_stats()["kernel_ms"]
+    ctx.synchronize()
+    plain_wall = time.perf_counter() - t1
+    value = n_total * steps / wall
+    plain_value = n_total * steps / plain_wall
+    algo = sum(bytes_per_source(h) * h.n for h in hgs)
+    achieved = algo * steps / (sum(kernel_ms) / 1e3) / 1e9
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "traffic_cohort64.json")
+    if os.path.exists(prof):
+        with open(prof) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    res = {
+        "metric": METRIC, "value": round(value, 3), "unit": "sources/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / steps, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "cohort64", "graphs": 64, "nodes": "U[1000,20000]", "mean_out_degree": "U[10,60]",
+                   "weights": "f32 lognormal(0,0.5)", "sources": n_total, "arcs": m_total,
+                   "outputs": "per-source APSP summaries + betweenness of every graph in one replay"},
+        "gteps": round(arcs / wall / 1e9, 3),
+        "kernel_ms_per_step": round(sum(kernel_ms) / steps, 3),
+        "plain_64_launches": {"value": round(plain_value, 3), "unit": "sources/s",
+                              "ms_per_step": round(plain_wall * 1e3 / steps, 3),
+                              "kernel_ms_per_step": round(plain_kernel / steps, 3)},
+        "cohort_vs_plain": round(value / plain_value, 3),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+    }
+    if not args.no_cpu:
+        print("[bench] timing the CPU baseline", file=sys.stderr, flush=True)
+        res["cpu_baseline"] = cpu_baseline_cohort(hgs, args.cpu_seconds)
+        res["speedup_vs_cpu"] = round(value / res["cpu_baseline"]["value"], 2)
+    print(json.dumps(res), flush=True)
+
+
+def cpu_baseline_cohort(hgs, seconds: float):
+    """The oracle on every member graph in turn, each on an equal share of a bounded time budget (sources of a
+    graph are independent); the rate is the cohort's total sources over the extrapolated total time."""
+    from oracle import oracle as orc
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    total_t, sampled = 0.0, 0
+    per_graph = seconds / len(hgs)
+    for h in hgs:
+        c = orc.Csr(h.n, h.offsets, h.targets, h.weights)
+        k = threads
+        while True:
+            src = np.arange(0, h.n, max(1, h.n // k), dtype=np.uint32)[:k]
+            t0 = time.perf_counter()
+            orc.apsp(c, sources=src, threads=threads)
+            orc.brandes(c, sources=src, threads=threads)
+            dt = time.perf_counter() - t0
+            if dt >= per_graph / 4 or k >= h.n:
+                break
+            k = min(h.n, k * 4)
+        total_t += dt * h.n / len(src)
+        sampled += len(src)
+    n_total = sum(h.n for h in hgs)
+    return {"value": round(n_total / total_t, 3), "unit": "sources/s", "cores": threads, "kind": "port",
+            "sample": f"{sampled} sources spread over all 64 graphs, each graph's time extrapolated to all its sources; "
+                      f"avg-apsp + avg-centrality (oracle/oracle.cpp, {threads} threads)"}
+
+
 def cpu_baseline(g, sources, seconds: float):
     """Oracle restatement of the reference's CPU path (APSP via PriorityQueueVariant + BrandesVariant), all host
     cores, on a bounded sample of the same sources (sources are independent, so the rate scales linearly)."""
@@ -93,7 +207,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0, help="timed steps (default: one pass over the workload's sources)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="spatial50k", choices=["fixed5k", "spatial50k", "spatial200k"])
+    ap.add_argument("--workload", default="spatial50k", choices=["fixed5k", "spatial50k", "spatial200k", "cohort64"])
     ap.add_argument("--batch", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -108,6 +222,9 @@ def main():
 
     import neurograph_amd as ng
 
+    if args.workload == "cohort64":
+        run_cohort(args)
+        return
     t0 = time.perf_counter()
     hg, sources, batch, cfg = make_workload(args.workload)
     gen_s = time.perf_counter() - t0

@@ -152,9 +152,9 @@ ng_status ng_autotune_delta(ng_graph* g, ng_length_kind lkind, int mode, const u
 ng_status ng_graph_last_stats(const ng_graph* g, ng_call_stats* out);
 
 /* Cohort: many graphs of uneven size, all sources of all graphs, in one persistent launch captured in
- * a hipGraph and replayed. Each workgroup takes a contiguous, cost-balanced range of the (graph, source)
- * items; betweenness partials are flushed per (workgroup, graph) and summed in workgroup order, so the
- * results are deterministic. Member graphs are borrowed (they must outlive the cohort), must share one
+ * a hipGraph and replayed. Workgroup b runs the (graph, source) items b, b + S, b + 2S, ... in graph
+ * order, so all workgroups sweep the member graphs together; betweenness partials are flushed per
+ * (workgroup, graph) and summed in workgroup order, so the results are deterministic. Member graphs are borrowed (they must outlive the cohort), must share one
  * distance type and have no zero-length arcs. No reference counterpart (the reference runs one graph per
  * process); each member's results equal ng_brandes_accumulate over all its sources. */
 ng_status ng_cohort_create(ng_context* ctx, ng_graph* const* graphs, uint32_t n_graphs, ng_cohort** out);
@@ -164,9 +164,8 @@ void ng_cohort_destroy(ng_cohort* c);
  * graph order; kernel_ms (optional): duration of the whole replay (HIP events). */
 ng_status ng_cohort_brandes(ng_cohort* c, double* bc_out, ng_sssp_summary* summaries_out, double* kernel_ms);
 /* Autotuner of the cohort launch: per member graph the bucket width (ng_autotune_delta in Brandes mode on
- * n_sample evenly spaced sources) and the measured per-source cost that balances the static partition;
- * then the resident workgroups per CU (occupancy maximum and two lower values, one timed replay each).
- * The next ng_cohort_brandes recaptures with the chosen shapes. */
+ * n_sample evenly spaced sources), then the resident workgroups per CU (occupancy maximum and two lower
+ * values, one timed replay each). The next ng_cohort_brandes recaptures with the chosen shapes. */
 ng_status ng_cohort_autotune(ng_cohort* c, uint32_t n_sample);
 /* Stats of the last cohort replay (kernel_ms covers the whole replay). */
 ng_status ng_cohort_last_stats(const ng_cohort* c, ng_call_stats* out);

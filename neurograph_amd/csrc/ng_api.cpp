@@ -85,6 +85,7 @@ struct ng_context {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int cus = 0;
+    Workspace ws;
 };
 
 struct ng_graph {
@@ -97,7 +98,6 @@ struct ng_graph {
     std::vector<unsigned char> w;  // raw weights
     bool has_negative = false;
     View views[2];                 // [0] |w|, [1] 1/|w|
-    Workspace ws;
     ng_call_stats last{};
 };
 
@@ -111,14 +111,12 @@ struct ng_cohort {
     uint32_t total = 0;                // sum of member sizes = number of (graph, source) items
     std::vector<uint32_t> prefix;      // [G+1] item / node prefix per graph
     std::vector<double> delta;         // per graph (0: the graph's tuned or heuristic width)
-    std::vector<double> cost;          // per-source cost weight per graph (balances the static partition)
     int bpc = 0;                       // resident workgroups per CU (0: occupancy maximum)
     uint32_t slots = 0;
-    Workspace ws;
+    uint64_t ws_generation = 0;        // context workspace generation the capture was made against
     CohortGraph* d_gtab = nullptr;
     uint32_t* d_lend = nullptr;        // light-prefix ends of every member for its own width
     uint32_t* d_prefix = nullptr;
-    uint32_t* d_chunk = nullptr;
     uint64_t* d_fbase = nullptr;
     uint32_t* d_seg_ptr = nullptr;
     uint64_t* d_seg_off = nullptr;
@@ -305,7 +303,9 @@ void free_ws(Workspace& w) {
     dfree(w.bc_out);
     dfree(w.grad1);
     dfree(w.grad2);
+    const uint64_t gen = w.generation;
     w = Workspace{};
+    w.generation = gen + 1;
 }
 
 // Pile capacity per list: n first parkings plus 2n re-parkings with improved values.
@@ -333,12 +333,31 @@ uint32_t log_capacity(uint32_t n, uint64_t m) {
     return static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>(m, 1), want));
 }
 
-// Makes sure workspace w holds `slots` slots for graphs of up to n nodes and m arcs, with every array `mode`
-// needs and dist in kind dk.
+// Pile entries per list for a slot stride of n nodes; NG_PILE_CAPACITY shrinks it (tests use it to force the
+// exact bucket-advance path).
+uint32_t wanted_pcap(uint32_t n) {
+    uint32_t p = pile_capacity(n);
+    if (const char* pc = std::getenv("NG_PILE_CAPACITY")) {
+        const long long c = std::atoll(pc);
+        if (c > 0) p = static_cast<uint32_t>(std::max<long long>(c, static_cast<long long>(n) + 2 * kBlock));
+    }
+    return p;
+}
+
+// Makes sure workspace w holds at least `slots` slots for graphs of up to n nodes and m arcs, with every
+// array `mode` needs and dist in kind dk. Growing keeps the larger of the old and new dimensions.
 ng_status ensure_ws(Workspace& w, hipStream_t st, uint32_t n, uint64_t m, uint32_t slots, int mode, DKind dk, uint32_t n_sources) {
-    const int need_bits = (mode == MODE_SSSP) ? 0 : (MODE_BC | MODE_ARC1 | MODE_ARC2);
-    const bool realloc = w.slots < slots || w.n != n || w.m < m || dk_size(dk) > w.dist_bytes || (need_bits & ~w.mode_bits) != 0;
+    int need_bits = (mode == MODE_SSSP) ? 0 : (MODE_BC | MODE_ARC1 | MODE_ARC2);
+    const bool realloc = w.slots < slots || w.n < n || w.m < m || dk_size(dk) > w.dist_bytes || (need_bits & ~w.mode_bits) != 0 ||
+                         w.pcap != wanted_pcap(std::max(w.n, n));
     if (realloc) {
+        if (w.dist != nullptr) {
+            n = std::max(n, w.n);
+            m = std::max(m, w.m);
+            slots = std::max(slots, w.slots);
+            n_sources = std::max(n_sources, w.summaries_cap);
+            need_bits |= w.mode_bits;
+        }
         free_ws(w);
         const size_t ns = static_cast<size_t>(slots) * n;
         w.slots = slots;
@@ -351,12 +370,7 @@ ng_status ensure_ws(Workspace& w, hipStream_t st, uint32_t n, uint64_t m, uint32
         NG_HIP(hipMalloc(&w.sstamp, 4 * std::max<size_t>(ns, 1)));
         NG_HIP(hipMalloc(&w.pstamp, 4 * std::max<size_t>(ns, 1)));
         NG_HIP(hipMalloc(&w.lists, 12 * std::max<size_t>(ns, 1)));
-        w.pcap = pile_capacity(n);
-        // NG_PILE_CAPACITY shrinks the pile (tests use it to force the exact bucket-advance path)
-        if (const char* pc = std::getenv("NG_PILE_CAPACITY")) {
-            const long long c = std::atoll(pc);
-            if (c > 0) w.pcap = static_cast<uint32_t>(std::max<long long>(c, static_cast<long long>(n) + 2 * kBlock));
-        }
+        w.pcap = wanted_pcap(n);
         NG_HIP(hipMalloc(&w.pile, 16ull * 2 * w.pcap * slots));
         NG_HIP(hipMalloc(&w.counters, 16 * static_cast<size_t>(slots)));
         NG_HIP(launch_fill_u32(w.fstamp, ns, 0u, st));
@@ -398,11 +412,13 @@ ng_status ensure_ws(Workspace& w, hipStream_t st, uint32_t n, uint64_t m, uint32
         w.summaries = nullptr;
         NG_HIP(hipMalloc(&w.summaries, sizeof(ng_sssp_summary) * std::max<uint32_t>(n_sources, 1)));
         w.summaries_cap = n_sources;
+        ++w.generation;
     }
     if (w.sources_cap < n_sources) {
         dfree(w.sources);
         NG_HIP(hipMalloc(&w.sources, sizeof(uint32_t) * std::max<uint32_t>(n_sources, 1)));
         w.sources_cap = n_sources;
+        ++w.generation;
     }
     return NG_OK;
 }
@@ -458,19 +474,21 @@ ng_status run_batch(ng_graph* g, const RunSpec& rs, uint32_t* slots_used, double
     slots = std::min<uint64_t>(slots, std::max<uint32_t>(rs.n_sources, 1));
     size_t free_b = 0, total_b = 0;
     NG_HIP(hipMemGetInfo(&free_b, &total_b));
-    const uint64_t per = slot_bytes(g->n, g->m, rs.mode, 8, log_capacity(g->n, g->m));
-    // memory already held by this graph's workspace counts as available
-    const uint64_t held = static_cast<uint64_t>(g->ws.slots) * per;
+    Workspace& w = ctx->ws;
+    const uint32_t wn = std::max(w.n, g->n);
+    const uint64_t wm = std::max(w.m, g->m);
+    const uint64_t per = slot_bytes(wn, wm, rs.mode, 8, log_capacity(wn, wm));
+    // memory already held by the context's workspace counts as available
+    const uint64_t held = static_cast<uint64_t>(w.slots) * slot_bytes(w.n, w.m, rs.mode, 8, log_capacity(w.n, w.m));
     const uint64_t budget = static_cast<uint64_t>(static_cast<double>(free_b + held) * 0.6);
     if (per > 0) slots = std::min<uint64_t>(slots, std::max<uint64_t>(1, budget / per));
-    st = ensure_ws(g->ws, ctx->stream, g->n, g->m, static_cast<uint32_t>(slots), rs.mode, v.dk, rs.n_sources);
+    st = ensure_ws(w, ctx->stream, g->n, g->m, static_cast<uint32_t>(slots), rs.mode, v.dk, rs.n_sources);
     if (st != NG_OK) return st;
-    Workspace& w = g->ws;
     const hipStream_t s = ctx->stream;
     if (rs.n_sources) NG_HIP(hipMemcpyAsync(w.sources, rs.sources, sizeof(uint32_t) * rs.n_sources, hipMemcpyHostToDevice, s));
     NG_HIP(hipMemsetAsync(w.arcs_reached, 0, 8 * 32, s));
     NG_HIP(hipMemsetAsync(w.err, 0, 4, s));
-    if (rs.mode & MODE_BC) NG_HIP(launch_fill_f64(w.bc_slot, static_cast<uint64_t>(slots) * g->n, 0.0, s));
+    if (rs.mode & MODE_BC) NG_HIP(launch_fill_f64(w.bc_slot, static_cast<uint64_t>(slots) * w.n, 0.0, s));
     if (rs.mode & MODE_ARC1) NG_HIP(launch_fill_f64(w.grad1, g->m, 0.0, s));
     if (rs.mode & MODE_ARC2) NG_HIP(launch_fill_f64(w.grad2, g->m, 0.0, s));
     void* d_dist_out = nullptr;
@@ -544,7 +562,7 @@ ng_status check_graph(const ng_graph* g) {
 
 ng_status copy_summaries(ng_graph* g, uint32_t n_sources, ng_sssp_summary* out) {
     if (!out || !n_sources) return NG_OK;
-    NG_HIP(hipMemcpy(out, g->ws.summaries, sizeof(ng_sssp_summary) * n_sources, hipMemcpyDeviceToHost));
+    NG_HIP(hipMemcpy(out, g->ctx->ws.summaries, sizeof(ng_sssp_summary) * n_sources, hipMemcpyDeviceToHost));
     return NG_OK;
 }
 
@@ -631,6 +649,8 @@ void ng_context_destroy(ng_context* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    free_ws(ctx->ws);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -683,7 +703,6 @@ void ng_graph_destroy(ng_graph* g) {
     (void)hipStreamSynchronize(g->ctx->stream);
     free_view(g->views[0]);
     free_view(g->views[1]);
-    free_ws(g->ws);
     delete g;
 }
 
@@ -728,9 +747,10 @@ ng_status ng_brandes_accumulate(ng_graph* g, const uint32_t* sources, uint32_t n
     ng_status st = run_dependency(g, sources, n_sources, delta, MODE_BC, out_opt);
     if (st != NG_OK) return st;
     if (!n_sources) return NG_OK;
-    NG_HIP(launch_reduce_slots(g->ws.bc_slot, g->last.slots, g->n, g->ws.bc_out, g->ctx->stream));
+    Workspace& w = g->ctx->ws;
+    NG_HIP(launch_reduce_slots(w.bc_slot, g->last.slots, g->n, w.n, w.bc_out, g->ctx->stream));
     std::vector<double> tmp(g->n);
-    NG_HIP(hipMemcpyAsync(tmp.data(), g->ws.bc_out, 8 * static_cast<size_t>(g->n), hipMemcpyDeviceToHost, g->ctx->stream));
+    NG_HIP(hipMemcpyAsync(tmp.data(), w.bc_out, 8 * static_cast<size_t>(g->n), hipMemcpyDeviceToHost, g->ctx->stream));
     NG_HIP(hipStreamSynchronize(g->ctx->stream));
     for (uint32_t v = 0; v < g->n; ++v) bc_inout[v] += tmp[v];
     return NG_OK;
@@ -744,7 +764,7 @@ ng_status ng_edge_dependency(ng_graph* g, const uint32_t* sources, uint32_t n_so
     if (st != NG_OK) return st;
     if (!n_sources) return NG_OK;
     std::vector<double> tmp(g->m);
-    if (g->m) NG_HIP(hipMemcpy(tmp.data(), g->ws.grad1, 8 * g->m, hipMemcpyDeviceToHost));
+    if (g->m) NG_HIP(hipMemcpy(tmp.data(), g->ctx->ws.grad1, 8 * g->m, hipMemcpyDeviceToHost));
     for (uint64_t a = 0; a < g->m; ++a) edge_inout[a] += tmp[a];
     return NG_OK;
 }
@@ -759,8 +779,8 @@ ng_status ng_apsp_adjoint(ng_graph* g, const uint32_t* sources, uint32_t n_sourc
     if (!n_sources) return NG_OK;
     std::vector<double> t1(g->m), t2(g->m);
     if (g->m) {
-        NG_HIP(hipMemcpy(t1.data(), g->ws.grad1, 8 * g->m, hipMemcpyDeviceToHost));
-        NG_HIP(hipMemcpy(t2.data(), g->ws.grad2, 8 * g->m, hipMemcpyDeviceToHost));
+        NG_HIP(hipMemcpy(t1.data(), g->ctx->ws.grad1, 8 * g->m, hipMemcpyDeviceToHost));
+        NG_HIP(hipMemcpy(t2.data(), g->ctx->ws.grad2, 8 * g->m, hipMemcpyDeviceToHost));
     }
     // d|w|/dw = sign(w); the efficiency sum decreases with every length on a shortest path
     for (uint64_t a = 0; a < g->m; ++a) {
@@ -829,7 +849,6 @@ void cohort_free_device(ng_cohort* c) {
     dfree(c->d_gtab);
     dfree(c->d_lend);
     dfree(c->d_prefix);
-    dfree(c->d_chunk);
     dfree(c->d_fbase);
     dfree(c->d_seg_ptr);
     dfree(c->d_seg_off);
@@ -846,9 +865,9 @@ ng_status upload(T** dst, const std::vector<T>& src, hipStream_t st) {
     return NG_OK;
 }
 
-// Sizes, partitions and captures the cohort launch:
-//   * items = (graph g, source s) in graph order; slot b gets the contiguous item range chunk[b]..chunk[b+1],
-//     cut so that every slot carries the same estimated cost (per-source cost weight of each graph);
+// Sizes and captures the cohort launch:
+//   * items = (graph g, source s) in graph order; slot b runs items b, b + S, b + 2S, ... (S slots), so all
+//     slots work on the same member graph at a time;
 //   * each slot flushes its betweenness partials once per graph it touches, into its own flush region;
 //     the reduction sums a graph's regions in slot order, so results are deterministic;
 //   * the captured hipGraph holds fill kernels (no memset nodes), the batched kernel and the reduction.
@@ -898,44 +917,35 @@ ng_status cohort_build(ng_cohort* c) {
     slots = std::min<uint64_t>(slots, std::max<uint32_t>(c->total, 1));
     size_t free_b = 0, total_b = 0;
     NG_HIP(hipMemGetInfo(&free_b, &total_b));
-    const uint64_t per = slot_bytes(c->n_max, c->m_max, MODE_BC, 8, log_capacity(c->n_max, c->m_max)) + 16ull * c->n_max;
-    const uint64_t held = static_cast<uint64_t>(c->ws.slots) * per;
+    Workspace& w = ctx->ws;
+    const uint32_t wn = std::max(w.n, c->n_max);
+    const uint64_t wm = std::max(w.m, c->m_max);
+    const uint64_t per = slot_bytes(wn, wm, MODE_BC, 8, log_capacity(wn, wm)) + 16ull * wn;
+    const uint64_t held = static_cast<uint64_t>(w.slots) * slot_bytes(w.n, w.m, MODE_BC, 8, log_capacity(w.n, w.m));
     const uint64_t budget = static_cast<uint64_t>(static_cast<double>(free_b + held) * 0.5);
     slots = std::min<uint64_t>(slots, std::max<uint64_t>(1, budget / per));
-    c->slots = static_cast<uint32_t>(slots);
-    // cost-balanced contiguous partition of the items
-    std::vector<uint32_t> chunk(slots + 1, c->total);
-    {
-        double C = 0.0;
-        for (uint32_t gi = 0; gi < G; ++gi) C += static_cast<double>(c->graphs[gi]->n) * c->cost[gi];
-        chunk[0] = 0;
-        uint64_t b = 1;
-        double acc = 0.0;
-        for (uint32_t gi = 0; gi < G && b < slots; ++gi) {
-            for (uint32_t i = 0; i < c->graphs[gi]->n && b < slots; ++i) {
-                acc += c->cost[gi];
-                while (b < slots && acc >= C * static_cast<double>(b) / static_cast<double>(slots)) chunk[b++] = c->prefix[gi] + i + 1;
+    // flush layout: slot b touches graph g when one of its items b + kS falls into g's item range
+    auto flush_layout = [&](uint64_t S, std::vector<uint64_t>* fbase, std::vector<std::vector<uint64_t>>* segs) {
+        uint64_t F = 0;
+        for (uint64_t b = 0; b < S; ++b) {
+            if (fbase) (*fbase)[b] = F;
+            for (uint32_t gi = 0; gi < G; ++gi) {
+                const uint64_t p0 = c->prefix[gi], p1 = c->prefix[gi + 1];
+                const uint64_t first = p0 + (b + S - p0 % S) % S;
+                if (first < p1) {
+                    if (segs) (*segs)[gi].push_back(F);
+                    F += p1 - p0;
+                }
             }
         }
-        while (b < slots) chunk[b++] = c->total;
-        chunk[slots] = c->total;
-    }
-    // flush layout
-    auto graph_of = [&](uint32_t item) {
-        return static_cast<uint32_t>(std::upper_bound(c->prefix.begin(), c->prefix.end(), item) - c->prefix.begin() - 1);
+        return F;
     };
+    // the flush regions take up to slots * total doubles: bound them by the memory budget too
+    while (slots > 1 && (slots * per + 8 * flush_layout(slots, nullptr, nullptr)) > budget) slots = slots * 7 / 8;
+    c->slots = static_cast<uint32_t>(slots);
     std::vector<uint64_t> fbase(slots, 0);
     std::vector<std::vector<uint64_t>> segs(G);
-    uint64_t F = 0;
-    for (uint64_t b = 0; b < slots; ++b) {
-        fbase[b] = F;
-        if (chunk[b] >= chunk[b + 1]) continue;
-        const uint32_t g0 = graph_of(chunk[b]), g1 = graph_of(chunk[b + 1] - 1);
-        for (uint32_t gi = g0; gi <= g1; ++gi) {
-            segs[gi].push_back(F);
-            F += c->graphs[gi]->n;
-        }
-    }
+    const uint64_t F = flush_layout(slots, &fbase, &segs);
     std::vector<uint32_t> seg_ptr(G + 1, 0);
     std::vector<uint64_t> seg_off;
     for (uint32_t gi = 0; gi < G; ++gi) {
@@ -944,16 +954,16 @@ ng_status cohort_build(ng_cohort* c) {
     }
     ng_status s1 = upload(&c->d_gtab, gtab, st);
     if (s1 == NG_OK) s1 = upload(&c->d_prefix, c->prefix, st);
-    if (s1 == NG_OK) s1 = upload(&c->d_chunk, chunk, st);
     if (s1 == NG_OK) s1 = upload(&c->d_fbase, fbase, st);
     if (s1 == NG_OK) s1 = upload(&c->d_seg_ptr, seg_ptr, st);
     if (s1 == NG_OK) s1 = upload(&c->d_seg_off, seg_off, st);
     if (s1 != NG_OK) return s1;
     NG_HIP(hipMalloc(&c->d_flush, 8 * std::max<uint64_t>(F, 1)));
     NG_HIP(hipMalloc(&c->d_bc, 8 * std::max<uint32_t>(c->total, 1)));
-    ng_status s2 = ensure_ws(c->ws, st, c->n_max, c->m_max, c->slots, MODE_BC, c->dk, c->total);
+    ng_status s2 = ensure_ws(w, st, c->n_max, c->m_max, c->slots, MODE_BC, c->dk, c->total);
     if (s2 != NG_OK) return s2;
     NG_HIP(hipStreamSynchronize(st));
+    c->ws_generation = w.generation;
 
     LaunchArgs la{};
     la.dk = c->dk;
@@ -964,17 +974,15 @@ ng_status cohort_build(ng_cohort* c) {
     la.n_sources = c->total;
     la.delta = 0.0;
     la.grid = c->slots;
-    la.ws = &c->ws;
+    la.ws = &w;
     la.dist_out = nullptr;
     la.dist_out_kind = 0;
     la.sorted = sorted;
     la.gtab = c->d_gtab;
     la.n_graphs = G;
     la.node_prefix = c->d_prefix;
-    la.chunk = c->d_chunk;
     la.flush = c->d_flush;
     la.flush_base = c->d_fbase;
-    Workspace& w = c->ws;
     NG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     hipError_t e = launch_fill_f64(w.bc_slot, static_cast<uint64_t>(c->slots) * w.n, 0.0, st);
     if (e == hipSuccess) e = launch_fill_u64(w.arcs_reached, 32, 0, st);
@@ -998,6 +1006,13 @@ ng_status cohort_replay(ng_cohort* c) {
     ng_context* ctx = c->ctx;
     const hipStream_t st = ctx->stream;
     NG_HIP(hipSetDevice(ctx->device));
+    // the context workspace is shared: plain calls in between may have grown it (new buffers: re-capture) or
+    // left +infinity of another distance kind in it (refilled here, outside the capture)
+    if (c->built) {
+        ng_status s0 = ensure_ws(ctx->ws, st, c->n_max, c->m_max, c->slots, MODE_BC, c->dk, c->total);
+        if (s0 != NG_OK) return s0;
+        if (ctx->ws.generation != c->ws_generation) c->built = false;
+    }
     if (!c->built) {
         ng_status s0 = cohort_build(c);
         if (s0 != NG_OK) return s0;
@@ -1011,8 +1026,8 @@ ng_status cohort_replay(ng_cohort* c) {
     NG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     uint32_t err = 0;
     unsigned long long ctrs[32] = {};
-    NG_HIP(hipMemcpy(&err, c->ws.err, 4, hipMemcpyDeviceToHost));
-    NG_HIP(hipMemcpy(ctrs, c->ws.arcs_reached, 8 * 32, hipMemcpyDeviceToHost));
+    NG_HIP(hipMemcpy(&err, ctx->ws.err, 4, hipMemcpyDeviceToHost));
+    NG_HIP(hipMemcpy(ctrs, ctx->ws.arcs_reached, 8 * 32, hipMemcpyDeviceToHost));
     if (err & ERR_KAHN) return fail(NG_EDEVICE, "tight-arc DAG has a cycle (zero-length arcs?)");
     if (err & ERR_GUARD) return fail(NG_EDEVICE, "iteration guard hit in the batched kernel");
     if (err & ERR_OVERFLOW) return fail(NG_EDEVICE, "queue overflow in the batched kernel");
@@ -1050,8 +1065,6 @@ ng_status ng_cohort_create(ng_context* ctx, ng_graph* const* graphs, uint32_t n_
         if (!graphs[i] || graphs[i]->ctx != ctx) return fail(NG_EINVAL, "graph %u is null or belongs to another context", i);
         c->graphs.push_back(graphs[i]);
         c->delta.push_back(0.0);
-        // default cost model of one source: a pass over the arcs and the nodes it reaches
-        c->cost.push_back(static_cast<double>(graphs[i]->m) + 2.0 * graphs[i]->n + 64.0);
     }
     *out = c.release();
     return NG_OK;
@@ -1062,7 +1075,6 @@ void ng_cohort_destroy(ng_cohort* c) {
     (void)hipSetDevice(c->ctx->device);
     (void)hipStreamSynchronize(c->ctx->stream);
     cohort_free_device(c);
-    free_ws(c->ws);
     delete c;
 }
 
@@ -1073,7 +1085,7 @@ ng_status ng_cohort_brandes(ng_cohort* c, double* bc_out, ng_sssp_summary* summa
     if (st != NG_OK) return st;
     NG_HIP(hipMemcpy(bc_out, c->d_bc, 8 * static_cast<size_t>(c->total), hipMemcpyDeviceToHost));
     if (summaries_out && c->total) {
-        NG_HIP(hipMemcpy(summaries_out, c->ws.summaries, sizeof(ng_sssp_summary) * c->total, hipMemcpyDeviceToHost));
+        NG_HIP(hipMemcpy(summaries_out, c->ctx->ws.summaries, sizeof(ng_sssp_summary) * c->total, hipMemcpyDeviceToHost));
     }
     if (kernel_ms) *kernel_ms = c->last.kernel_ms;
     return NG_OK;
@@ -1082,7 +1094,7 @@ ng_status ng_cohort_brandes(ng_cohort* c, double* bc_out, ng_sssp_summary* summa
 ng_status ng_cohort_autotune(ng_cohort* c, uint32_t n_sample) {
     if (!c) return fail(NG_EINVAL, "null cohort");
     if (n_sample == 0) return fail(NG_EINVAL, "empty sample");
-    // per graph: the bucket width, then the slot-time of one source at that width (partition weight)
+    // per graph: the bucket width
     for (size_t gi = 0; gi < c->graphs.size(); ++gi) {
         ng_graph* g = c->graphs[gi];
         const uint32_t k = std::min(n_sample, g->n);
@@ -1092,12 +1104,6 @@ ng_status ng_cohort_autotune(ng_cohort* c, uint32_t n_sample) {
         ng_status st = ng_autotune_delta(g, NG_LEN_ABS, 1, sample.data(), k, &d);
         if (st != NG_OK) return st;
         c->delta[gi] = d;
-        RunSpec rs{0, MODE_BC, d, sample.data(), k, nullptr};
-        uint32_t used = 0;
-        st = run_batch(g, rs, &used, nullptr);
-        if (st != NG_OK) return st;
-        const double busy = static_cast<double>(std::min<uint32_t>(used, k));
-        c->cost[gi] = std::max(1e-6, g->last.kernel_ms * busy / static_cast<double>(k));
     }
     // resident workgroups per CU: the occupancy maximum and two lower values, one whole replay each
     c->bpc = 0;

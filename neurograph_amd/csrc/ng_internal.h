@@ -34,7 +34,9 @@ struct View {
     uint32_t* lend = nullptr;  // [n] end of the light prefix of each row for lend_delta
 };
 
-// Per-slot scratch of the batched kernel. Sized for (slots, n, m); grown on demand, never shrunk.
+// Per-slot scratch of the batched kernel, one per context, shared by its graphs and cohorts (calls on a
+// context are sequential). Sized for (slots, n, m) = the largest request so far; grown, never shrunk. A
+// slot's arrays have stride n; a smaller graph uses a prefix of each.
 struct Workspace {
     uint32_t slots = 0, n = 0;
     uint64_t m = 0;
@@ -71,6 +73,7 @@ struct Workspace {
     double* grad2 = nullptr;    // [m]
     DKind dist_kind = DKind::U32;  // element kind the dist array currently holds +infinity in
     bool dist_valid = false;
+    uint64_t generation = 0;       // bumped whenever a buffer is reallocated (captured launches must re-capture)
 };
 
 // Device error bits reported by the batched kernel.
@@ -111,7 +114,6 @@ struct LaunchArgs {
     const CohortGraph* gtab = nullptr;
     uint32_t n_graphs = 0;
     const uint32_t* node_prefix = nullptr;  // [n_graphs+1] item prefix (= node prefix) per graph
-    const uint32_t* chunk = nullptr;        // [grid+1] item range per slot
     double* flush = nullptr;
     const uint64_t* flush_base = nullptr;   // [grid] start of each slot's flush region
 };
@@ -120,7 +122,7 @@ struct LaunchArgs {
 hipError_t launch_batch(const LaunchArgs& a, hipStream_t stream);
 hipError_t launch_light_end(DKind dk, uint32_t n, const uint32_t* off, const void* len, double delta, uint32_t* lend,
                             hipStream_t stream);
-hipError_t launch_reduce_slots(const double* bc_slot, uint32_t slots, uint32_t n, double* out, hipStream_t stream);
+hipError_t launch_reduce_slots(const double* bc_slot, uint32_t slots, uint32_t n, uint32_t stride, double* out, hipStream_t stream);
 // out[node_prefix[g] + v] = sum over k in [seg_ptr[g], seg_ptr[g+1]) of flush[seg_off[k] + v] (slot order).
 hipError_t launch_cohort_reduce(const double* flush, const uint32_t* node_prefix, uint32_t n_graphs, const uint32_t* seg_ptr,
                                 const uint64_t* seg_off, uint32_t total_nodes, double* out, hipStream_t stream);

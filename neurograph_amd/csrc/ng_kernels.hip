@@ -370,7 +370,6 @@ struct KArgs {
     const CohortGraph* gtab;
     uint32_t n_graphs;
     const uint32_t* node_prefix;
-    const uint32_t* chunk;
     double* flush;
     const uint64_t* flush_base;
     uint32_t bm_words;  // words of the LDS bitmap of finalized nodes (0: no bitmap)
@@ -496,10 +495,11 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
     unsigned long long c_phases = 0, c_buckets = 0, c_light = 0, c_settle = 0, c_logged = 0, c_layers = 0;
     unsigned long long c_probe = 0, c_improve = 0;
 
-    // Plain launch: sources blockIdx.x, +gridDim.x, ... of a.sources. Cohort launch (a.gtab != nullptr): the
-    // contiguous item range [chunk[slot], chunk[slot+1]) of the cohort's (graph, source) items in graph order;
-    // when the graph changes, the slot's betweenness partials of the previous graph are flushed to the slot's
-    // flush region (and cleared), which a reduction sums in slot order.
+    // Sources blockIdx.x, +gridDim.x, ... of a.sources (plain launch), or of the cohort's (graph, source) items
+    // in graph order (cohort launch, a.gtab != nullptr): all slots sweep the member graphs together, so the
+    // CSR in use stays cache-resident and the tail of one graph overlaps the start of the next. When a slot's
+    // graph changes, its betweenness partials of the previous graph are flushed to the slot's flush region
+    // (and cleared); a reduction sums each graph's regions in slot order.
     const bool cohort = a.gtab != nullptr;
     uint32_t cur_g = 0xFFFFFFFFu;
     size_t fpos = cohort ? a.flush_base[slot] : 0;
@@ -512,10 +512,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
             fpos += n;
         }
     };
-    const uint32_t it_first = cohort ? a.chunk[slot] : blockIdx.x;
-    const uint32_t it_end = cohort ? a.chunk[slot + 1] : a.n_sources;
-    const uint32_t it_step = cohort ? 1u : gridDim.x;
-    for (uint32_t si = it_first; si < it_end; si += it_step) {
+    for (uint32_t si = blockIdx.x; si < a.n_sources; si += gridDim.x) {
         uint32_t s;
         if (cohort) {
             uint32_t lo = 0, hi = a.n_graphs;  // graph g holds items [node_prefix[g], node_prefix[g+1])
@@ -1062,12 +1059,12 @@ __global__ void light_end_kernel(uint32_t n, const uint32_t* off, const L* len, 
     lend[v] = lo;
 }
 
-// out[v] = sum over slots of part[slot][v], in slot order (deterministic)
-__global__ void reduce_slots_kernel(const double* part, uint32_t slots, uint32_t n, double* out) {
+// out[v] = sum over slots of part[slot * stride + v], in slot order (deterministic)
+__global__ void reduce_slots_kernel(const double* part, uint32_t slots, uint32_t n, uint32_t stride, double* out) {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
     double acc = 0.0;
-    for (uint32_t k = 0; k < slots; ++k) acc += part[static_cast<size_t>(k) * n + v];
+    for (uint32_t k = 0; k < slots; ++k) acc += part[static_cast<size_t>(k) * stride + v];
     out[v] = acc;
 }
 
@@ -1113,7 +1110,6 @@ hipError_t launch_typed(const LaunchArgs& la, hipStream_t stream) {
         k.gtab = la.gtab;
         k.n_graphs = la.n_graphs;
         k.node_prefix = la.node_prefix;
-        k.chunk = la.chunk;
         k.flush = la.flush;
         k.flush_base = la.flush_base;
     }
@@ -1260,9 +1256,9 @@ hipError_t launch_cohort_reduce(const double* flush, const uint32_t* node_prefix
     return hipGetLastError();
 }
 
-hipError_t launch_reduce_slots(const double* bc_slot, uint32_t slots, uint32_t n, double* out, hipStream_t stream) {
+hipError_t launch_reduce_slots(const double* bc_slot, uint32_t slots, uint32_t n, uint32_t stride, double* out, hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(reduce_slots_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, bc_slot, slots, n, out);
+    hipLaunchKernelGGL(reduce_slots_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, bc_slot, slots, n, stride, out);
     return hipGetLastError();
 }
 

@@ -104,3 +104,19 @@ def test_cohort_errors(ctx):
     z = ng.HostGraph.from_edges(3, [0, 1], [1, 2], np.array([0.0, 1.0], dtype=np.float32))
     with pytest.raises(ng.NgError, match="zero-length"):
         ng.Cohort(ctx, [g32, ng.Graph(ctx, z)]).brandes()
+
+
+def test_cohort_recaptures_after_workspace_growth(ctx):
+    """A plain call on a larger graph grows the context's shared workspace between replays; the next replay
+    re-captures against the new buffers and gives the same results."""
+    import neurograph_amd as ng
+
+    hgs = members(ng, np.float32, [500, 900], [8, 12], 150)
+    gs = [ng.Graph(ctx, h) for h in hgs]
+    co = ng.Cohort(ctx, gs)
+    first, _, _ = co.brandes()
+    large = ng.Graph(ctx, ng.generate_spatial(30000, 10, weights=np.float64, seed=151))
+    large.brandes(np.arange(0, 30000, 7))
+    again, _, _ = co.brandes()
+    for a, b in zip(first, again):
+        np.testing.assert_array_equal(a, b)

@@ -233,10 +233,13 @@ def test_fixed_degree_config_parity(ctx):
 
 def test_small_pile_forces_exact_bucket_advance(ctx, monkeypatch):
     """With the pile reserve exhausted, bucket advances reload distances instead of trusting parked values."""
-    monkeypatch.setenv("NG_PILE_CAPACITY", "1")  # clamped to n + 512 entries: every re-parking is refused
+    import neurograph_amd as ng
+
+    monkeypatch.setenv("NG_PILE_CAPACITY", "1")  # clamped to n + 512 entries: re-parking is refused almost always
+    own = ng.Context(0)  # a fresh context, so that the workspace stride is this graph's n
     for wdtype in (np.int32, np.float32):
         c = rand_graph(3000, 30000, 77, wdtype=wdtype, wlo=1, whi=20)
-        g = dev_graph(ctx, c)
+        g = dev_graph(own, c)
         srcs = np.arange(0, 3000, 37)
         ref = np.stack([orc.sssp(c, int(s))[0] for s in srcs])
         for delta in (0.0, 0.5, 3.0):
@@ -261,3 +264,19 @@ def test_settle_order_variants(ctx, monkeypatch, order):
             np.testing.assert_array_equal(dist, ref)
         bc, summ = g.brandes(srcs, summaries=True)
         np.testing.assert_allclose(bc, orc.brandes(c, sources=srcs)["bc"], rtol=1e-9, atol=1e-9)
+
+
+def test_shared_workspace_across_graphs(ctx):
+    """Graphs of one context share its workspace (grown to the largest graph, never shrunk): alternating a large
+    and a small graph, and distance kinds, keeps every result equal to the oracle's."""
+    big = rand_graph(6000, 50000, 31, wdtype=np.float64)
+    small = rand_graph(700, 5000, 32, wdtype=np.int32, wlo=1, whi=5)
+    mid = rand_graph(2500, 20000, 33, wdtype=np.float32)
+    gs = {"small": (small, dev_graph(ctx, small)), "big": (big, dev_graph(ctx, big)), "mid": (mid, dev_graph(ctx, mid))}
+    for name in ("small", "big", "small", "mid", "big", "small"):
+        c, g = gs[name]
+        srcs = np.arange(0, c.n, max(1, c.n // 90))
+        ref = np.stack([orc.sssp(c, int(s))[0] for s in srcs])
+        _, dist = g.sssp(srcs, want_dist=True)
+        np.testing.assert_array_equal(dist, ref)
+        np.testing.assert_allclose(g.brandes(srcs), orc.brandes(c, sources=srcs)["bc"], rtol=1e-9, atol=1e-9)
