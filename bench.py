@@ -15,6 +15,9 @@ Workloads (default spatial50k):
   cohort64     64 spatial graphs, nodes U[1k, 20k], mean out-degree U[10, 60], FP32 lognormal(0,0.5), all sources of all
                graphs; one step = one replay of the hipGraph-captured persistent cohort launch, reported beside the
                same work as 64 plain launches (one ng_brandes_accumulate per graph)
+  adjoint20k   20k nodes, mean out-degree 30, FP64 weights U[0.5, 1.5]; one step = the gradient of CPL and GE with
+               respect to every arc weight over all 20k sources (ng_apsp_adjoint), reported beside the plain
+               forward (ng_sssp_summaries) and Brandes (ng_brandes_accumulate) passes over the same sources
 
 Roofline: algorithmic bytes per source B_bc = 2*(A*(s_i+s_w) + (V+1)*s_o) + V*(2*s_d + 32) (one pass over the
 CSR for the forward sweep, one for the backward sweep, per-node distance/sigma/delta/betweenness state) divided
@@ -147,6 +150,53 @@ def run_cohort(args):
     print(json.dumps(res), flush=True)
 
 
+def run_adjoint(args):
+    import neurograph_amd as ng
+
+    hg = ng.generate_spatial(20000, 30, box=1000.0, lam=150.0, wdist="uniform", a=0.5, b=1.5, weights=np.float64, seed=5)
+    ctx = ng.Context(0)
+    g = ng.Graph(ctx, hg)
+    src = np.arange(hg.n, dtype=np.uint32)
+    sample = np.arange(0, hg.n, 40, dtype=np.uint32)
+    g.autotune(sample, mode=1)
+    g.autotune(sample, mode=0)
+    steps = args.steps or 3
+    for _ in range(args.warmup):
+        g.adjoint(src)
+    times = {}
+    kms = {}
+    for name, fn in (("adjoint", lambda: g.adjoint(src)), ("forward", lambda: g.sssp(src)),
+                     ("brandes", lambda: g.brandes(src))):
+        fn()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        k = 0.0
+        for _ in range(steps):
+            fn()
+            k += g.last_stats()["kernel_ms"]
+        ctx.synchronize()
+        times[name] = (time.perf_counter() - t0) / steps
+        kms[name] = k / steps
+    value = hg.n / times["adjoint"]
+    # adjoint traffic model: B_bc plus the gradient read-modify-write of the two per-arc vectors
+    algo = bytes_per_source(hg) * hg.n + 2 * 16.0 * hg.m
+    achieved = algo / (kms["adjoint"] / 1e3) / 1e9
+    res = {
+        "metric": METRIC, "value": round(value, 3), "unit": "sources/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": round(times["adjoint"] * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "adjoint20k", "nodes": hg.n, "arcs": hg.m, "mean_out_degree": 30, "weights": "f64 U[0.5,1.5]",
+                   "outputs": "dCPL/dw and dGE/dw for every arc, all sources"},
+        "kernel_ms_per_step": round(kms["adjoint"], 3),
+        "plain_forward": {"value": round(hg.n / times["forward"], 3), "unit": "sources/s", "kernel_ms": round(kms["forward"], 3)},
+        "plain_brandes": {"value": round(hg.n / times["brandes"], 3), "unit": "sources/s", "kernel_ms": round(kms["brandes"], 3)},
+        "adjoint_vs_forward": round(times["forward"] / times["adjoint"], 3),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+    }
+    print(json.dumps(res), flush=True)
+
+
 def cpu_baseline_cohort(hgs, seconds: float):
     """The oracle on every member graph in turn, each on an equal share of a bounded time budget (sources of a
     graph are independent); the rate is the cohort's total sources over the extrapolated total time."""
@@ -207,7 +257,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0, help="timed steps (default: one pass over the workload's sources)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="spatial50k", choices=["fixed5k", "spatial50k", "spatial200k", "cohort64"])
+    ap.add_argument("--workload", default="spatial50k",
+                    choices=["fixed5k", "spatial50k", "spatial200k", "cohort64", "adjoint20k"])
     ap.add_argument("--batch", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -224,6 +275,9 @@ def main():
 
     if args.workload == "cohort64":
         run_cohort(args)
+        return
+    if args.workload == "adjoint20k":
+        run_adjoint(args)
         return
     t0 = time.perf_counter()
     hg, sources, batch, cfg = make_workload(args.workload)
