@@ -281,20 +281,16 @@ void free_ws(Workspace& w) {
     w.dist = nullptr;
     dfree(w.fstamp);
     dfree(w.sstamp);
-    dfree(w.pstamp);
+    dfree(w.rec);
+    dfree(w.delta2);
     dfree(w.lists);
     if (w.pile) (void)hipFree(w.pile);
     w.pile = nullptr;
     dfree(w.counters);
-    dfree(w.sigma);
-    dfree(w.delta1);
-    dfree(w.delta2);
-    dfree(w.indeg);
     dfree(w.order);
     dfree(w.layers);
     dfree(w.bits);
     dfree(w.log);
-    dfree(w.bc_slot);
     if (w.summaries) (void)hipFree(w.summaries);
     w.summaries = nullptr;
     dfree(w.arcs_reached);
@@ -308,7 +304,6 @@ void free_ws(Workspace& w) {
     w.generation = gen + 1;
 }
 
-// Pile capacity per list: n first parkings plus 2n re-parkings with improved values.
 // Settle passes expand each bucket's settled nodes in increasing id (a second LDS bitmap) when that
 // costs no resident workgroups: measured +5 % on a 50k-node spatial graph, but a loss wherever the larger
 // LDS footprint lowers occupancy. NG_SORTED_SETTLE=0/1 overrides the choice (tuning experiments).
@@ -317,13 +312,17 @@ bool settle_sorted(DKind dk, int mode, uint32_t n) {
     return batch_blocks_per_cu(dk, mode, n, true) >= batch_blocks_per_cu(dk, mode, n, false);
 }
 
+// Pile capacity per list: n first parkings plus 2n re-parkings with improved values.
 uint32_t pile_capacity(uint32_t n) { return static_cast<uint32_t>(std::min<uint64_t>(3ull * n + 1024, 0xFFFFFF00u)); }
 
-// Per-slot device bytes of the batched kernel for (n, m, mode).
-uint64_t slot_bytes(uint32_t n, uint64_t m, int mode, size_t dbytes, uint32_t logcap) {
-    uint64_t b = static_cast<uint64_t>(n) * (dbytes + 12 + 12) + 16 + 32ull * pile_capacity(n);
+// Per-slot device bytes of the batched kernel for (n, m, mode): distances, frontier and settled stamps, the
+// three lists and the pile; in Brandes modes also the records, second dependency, Kahn order, layer bounds,
+// tight-arc bitmap and log.
+uint64_t slot_bytes(uint32_t n, uint64_t m, int mode, uint32_t logcap) {
+    uint64_t b = static_cast<uint64_t>(n) * (8 + 4 + 4 + 12) + 16 + 32ull * pile_capacity(n);
     if (mode != MODE_SSSP) {
-        b += static_cast<uint64_t>(n) * (8 + 8 + 4 + 4 + 4 + 8 + 8) + 4 + ((m + 31) / 32) * 4 + static_cast<uint64_t>(logcap) * 8;
+        b += static_cast<uint64_t>(n) * (sizeof(BcRec) + 8 + 4 + 4) + 4 + ((m + 31) / 32) * 4 +
+             static_cast<uint64_t>(logcap) * sizeof(LogEntry);
     }
     return b;
 }
@@ -363,39 +362,32 @@ ng_status ensure_ws(Workspace& w, hipStream_t st, uint32_t n, uint64_t m, uint32
         w.slots = slots;
         w.n = n;
         w.m = m;
-        w.dist_bytes = 8;  // room for either element size
         w.mode_bits = need_bits;
+        w.dist_bytes = 8;  // room for either element size
         NG_HIP(hipMalloc(&w.dist, 8 * std::max<size_t>(ns, 1)));
         NG_HIP(hipMalloc(&w.fstamp, 4 * std::max<size_t>(ns, 1)));
         NG_HIP(hipMalloc(&w.sstamp, 4 * std::max<size_t>(ns, 1)));
-        NG_HIP(hipMalloc(&w.pstamp, 4 * std::max<size_t>(ns, 1)));
         NG_HIP(hipMalloc(&w.lists, 12 * std::max<size_t>(ns, 1)));
         w.pcap = wanted_pcap(n);
         NG_HIP(hipMalloc(&w.pile, 16ull * 2 * w.pcap * slots));
         NG_HIP(hipMalloc(&w.counters, 16 * static_cast<size_t>(slots)));
         NG_HIP(launch_fill_u32(w.fstamp, ns, 0u, st));
         NG_HIP(launch_fill_u32(w.sstamp, ns, 0u, st));
-        NG_HIP(launch_fill_u32(w.pstamp, ns, 0u, st));
         NG_HIP(launch_fill_u32(w.counters, 4ull * slots, 1u, st));
         if (need_bits) {
             w.logcap = log_capacity(n, m);
             w.words = (m + 31) / 32;
-            NG_HIP(hipMalloc(&w.sigma, 8 * std::max<size_t>(ns, 1)));
-            NG_HIP(hipMalloc(&w.delta1, 8 * std::max<size_t>(ns, 1)));
+            NG_HIP(hipMalloc(&w.rec, sizeof(BcRec) * std::max<size_t>(ns, 1)));
             NG_HIP(hipMalloc(&w.delta2, 8 * std::max<size_t>(ns, 1)));
-            NG_HIP(hipMalloc(&w.indeg, 4 * std::max<size_t>(ns, 1)));
+            NG_HIP(launch_rec_init(w.rec, ns, st));
+            NG_HIP(launch_fill_f64(w.delta2, ns, 0.0, st));
             NG_HIP(hipMalloc(&w.order, 4 * std::max<size_t>(ns, 1)));
             NG_HIP(hipMalloc(&w.layers, 4 * static_cast<size_t>(slots) * (n + 1)));
             NG_HIP(hipMalloc(&w.bits, 4 * std::max<uint64_t>(w.words * slots, 1)));
-            NG_HIP(hipMalloc(&w.log, 8 * static_cast<size_t>(w.logcap) * slots));
-            NG_HIP(hipMalloc(&w.bc_slot, 8 * std::max<size_t>(ns, 1)));
+            NG_HIP(hipMalloc(&w.log, sizeof(LogEntry) * static_cast<size_t>(w.logcap) * slots));
             NG_HIP(hipMalloc(&w.bc_out, 8 * std::max<uint32_t>(n, 1)));
             NG_HIP(hipMalloc(&w.grad1, 8 * std::max<uint64_t>(m, 1)));
             NG_HIP(hipMalloc(&w.grad2, 8 * std::max<uint64_t>(m, 1)));
-            NG_HIP(launch_fill_f64(w.sigma, ns, 0.0, st));
-            NG_HIP(launch_fill_f64(w.delta1, ns, 0.0, st));
-            NG_HIP(launch_fill_f64(w.delta2, ns, 0.0, st));
-            NG_HIP(launch_fill_u32(w.indeg, ns, 0u, st));
             NG_HIP(launch_fill_u32(w.bits, w.words * slots, 0u, st));
         }
         NG_HIP(hipMalloc(&w.arcs_reached, 8 * 32));
@@ -477,9 +469,9 @@ ng_status run_batch(ng_graph* g, const RunSpec& rs, uint32_t* slots_used, double
     Workspace& w = ctx->ws;
     const uint32_t wn = std::max(w.n, g->n);
     const uint64_t wm = std::max(w.m, g->m);
-    const uint64_t per = slot_bytes(wn, wm, rs.mode, 8, log_capacity(wn, wm));
+    const uint64_t per = slot_bytes(wn, wm, rs.mode, log_capacity(wn, wm));
     // memory already held by the context's workspace counts as available
-    const uint64_t held = static_cast<uint64_t>(w.slots) * slot_bytes(w.n, w.m, rs.mode, 8, log_capacity(w.n, w.m));
+    const uint64_t held = static_cast<uint64_t>(w.slots) * slot_bytes(w.n, w.m, rs.mode, log_capacity(w.n, w.m));
     const uint64_t budget = static_cast<uint64_t>(static_cast<double>(free_b + held) * 0.6);
     if (per > 0) slots = std::min<uint64_t>(slots, std::max<uint64_t>(1, budget / per));
     st = ensure_ws(w, ctx->stream, g->n, g->m, static_cast<uint32_t>(slots), rs.mode, v.dk, rs.n_sources);
@@ -488,7 +480,7 @@ ng_status run_batch(ng_graph* g, const RunSpec& rs, uint32_t* slots_used, double
     if (rs.n_sources) NG_HIP(hipMemcpyAsync(w.sources, rs.sources, sizeof(uint32_t) * rs.n_sources, hipMemcpyHostToDevice, s));
     NG_HIP(hipMemsetAsync(w.arcs_reached, 0, 8 * 32, s));
     NG_HIP(hipMemsetAsync(w.err, 0, 4, s));
-    if (rs.mode & MODE_BC) NG_HIP(launch_fill_f64(w.bc_slot, static_cast<uint64_t>(slots) * w.n, 0.0, s));
+    if (rs.mode & MODE_BC) NG_HIP(launch_rec_zero_bc(w.rec, static_cast<uint64_t>(slots) * w.n, s));
     if (rs.mode & MODE_ARC1) NG_HIP(launch_fill_f64(w.grad1, g->m, 0.0, s));
     if (rs.mode & MODE_ARC2) NG_HIP(launch_fill_f64(w.grad2, g->m, 0.0, s));
     void* d_dist_out = nullptr;
@@ -748,7 +740,7 @@ ng_status ng_brandes_accumulate(ng_graph* g, const uint32_t* sources, uint32_t n
     if (st != NG_OK) return st;
     if (!n_sources) return NG_OK;
     Workspace& w = g->ctx->ws;
-    NG_HIP(launch_reduce_slots(w.bc_slot, g->last.slots, g->n, w.n, w.bc_out, g->ctx->stream));
+    NG_HIP(launch_reduce_slots(w.rec, g->last.slots, g->n, w.n, w.bc_out, g->ctx->stream));
     std::vector<double> tmp(g->n);
     NG_HIP(hipMemcpyAsync(tmp.data(), w.bc_out, 8 * static_cast<size_t>(g->n), hipMemcpyDeviceToHost, g->ctx->stream));
     NG_HIP(hipStreamSynchronize(g->ctx->stream));
@@ -920,8 +912,8 @@ ng_status cohort_build(ng_cohort* c) {
     Workspace& w = ctx->ws;
     const uint32_t wn = std::max(w.n, c->n_max);
     const uint64_t wm = std::max(w.m, c->m_max);
-    const uint64_t per = slot_bytes(wn, wm, MODE_BC, 8, log_capacity(wn, wm)) + 16ull * wn;
-    const uint64_t held = static_cast<uint64_t>(w.slots) * slot_bytes(w.n, w.m, MODE_BC, 8, log_capacity(w.n, w.m));
+    const uint64_t per = slot_bytes(wn, wm, MODE_BC, log_capacity(wn, wm)) + 16ull * wn;
+    const uint64_t held = static_cast<uint64_t>(w.slots) * slot_bytes(w.n, w.m, MODE_BC, log_capacity(w.n, w.m));
     const uint64_t budget = static_cast<uint64_t>(static_cast<double>(free_b + held) * 0.5);
     slots = std::min<uint64_t>(slots, std::max<uint64_t>(1, budget / per));
     // flush layout: slot b touches graph g when one of its items b + kS falls into g's item range
@@ -984,7 +976,7 @@ ng_status cohort_build(ng_cohort* c) {
     la.flush = c->d_flush;
     la.flush_base = c->d_fbase;
     NG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    hipError_t e = launch_fill_f64(w.bc_slot, static_cast<uint64_t>(c->slots) * w.n, 0.0, st);
+    hipError_t e = launch_rec_zero_bc(w.rec, static_cast<uint64_t>(c->slots) * w.n, st);
     if (e == hipSuccess) e = launch_fill_u64(w.arcs_reached, 32, 0, st);
     if (e == hipSuccess) e = launch_fill_u32(w.err, 1, 0u, st);
     if (e == hipSuccess) e = launch_batch(la, st);

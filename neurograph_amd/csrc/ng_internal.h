@@ -34,33 +34,48 @@ struct View {
     uint32_t* lend = nullptr;  // [n] end of the light prefix of each row for lend_delta
 };
 
+// Betweenness state of one node in a slot, co-located so that a random touch of a node in the sigma layers
+// (sigma + in-degree) and the dependency layers (sigma + dependency + betweenness partial) costs one cache
+// line. Distances and frontier stamps stay in compact arrays of their own: the relaxation passes probe them
+// far more often, and compact arrays keep more of them cache-resident.
+struct alignas(32) BcRec {
+    double sigma;   // number of shortest paths
+    double delta1;  // dependency (seed 1)
+    double bc;      // betweenness partial sum of the slot (kept across sources)
+    uint32_t indeg; // tight in-degree still to be released (sigma layers)
+    uint32_t spare;
+};
+static_assert(sizeof(BcRec) == 32, "BcRec must be a quarter of a 128-byte line");
+
+// A candidate tight arc logged by a settle pass: arc e into t with candidate distance nd (bits).
+struct LogEntry {
+    uint32_t t;
+    uint32_t e;
+    unsigned long long nd;
+};
+
 // Per-slot scratch of the batched kernel, one per context, shared by its graphs and cohorts (calls on a
 // context are sequential). Sized for (slots, n, m) = the largest request so far; grown, never shrunk. A
 // slot's arrays have stride n; a smaller graph uses a prefix of each.
 struct Workspace {
     uint32_t slots = 0, n = 0;
     uint64_t m = 0;
-    size_t dist_bytes = 0;  // element size of dist (4 or 8)
     int mode_bits = 0;      // which optional arrays exist (MODE_* flags)
     uint32_t logcap = 0;
-    void* dist = nullptr;       // [slots][n]
-    uint32_t* fstamp = nullptr; // [slots][n]
-    uint32_t* sstamp = nullptr; // [slots][n]
-    uint32_t* pstamp = nullptr; // [slots][n]
+    void* dist = nullptr;       // [slots][n] distance bits (4 or 8 bytes)
+    uint32_t* fstamp = nullptr; // [slots][n] phase that last pushed the node to the frontier
+    uint32_t* sstamp = nullptr; // [slots][n] settled stamps (used when the LDS bitmap does not fit)
+    BcRec* rec = nullptr;       // [slots][n] betweenness state (Brandes modes)
+    double* delta2 = nullptr;   // [slots][n] dependency seeded with 1/d^2 (adjoint mode)
     uint32_t* lists = nullptr;  // [slots][3][n]: frontier A/B, settled
     void* pile = nullptr;       // [slots][2][pcap] pile entries (16 bytes reserved per entry)
     uint32_t pcap = 0;
     uint32_t* counters = nullptr;  // [slots][4] persistent stamp generations
-    double* sigma = nullptr;    // [slots][n]
-    double* delta1 = nullptr;   // [slots][n]
-    double* delta2 = nullptr;   // [slots][n]
-    uint32_t* indeg = nullptr;  // [slots][n]
     uint32_t* order = nullptr;  // [slots][n]
     uint32_t* layers = nullptr; // [slots][n+1]
     uint32_t* bits = nullptr;   // [slots][words]
     uint64_t words = 0;
-    uint2* log = nullptr;       // [slots][logcap]
-    double* bc_slot = nullptr;  // [slots][n]
+    LogEntry* log = nullptr;    // [slots][logcap]
     void* summaries = nullptr;  // [max_sources] ng_sssp_summary
     uint32_t summaries_cap = 0;
     uint64_t* arcs_reached = nullptr;  // [16]: arcs reached, phases, buckets, light arcs, settle arcs, logged, layers,
@@ -71,7 +86,8 @@ struct Workspace {
     double* bc_out = nullptr;   // [n]
     double* grad1 = nullptr;    // [m]
     double* grad2 = nullptr;    // [m]
-    DKind dist_kind = DKind::U32;  // element kind the dist array currently holds +infinity in
+    size_t dist_bytes = 0;         // element size the dist array was allocated for
+    DKind dist_kind = DKind::U32;  // distance kind the dist array currently holds +infinity of
     bool dist_valid = false;
     uint64_t generation = 0;       // bumped whenever a buffer is reallocated (captured launches must re-capture)
 };
@@ -122,14 +138,18 @@ struct LaunchArgs {
 hipError_t launch_batch(const LaunchArgs& a, hipStream_t stream);
 hipError_t launch_light_end(DKind dk, uint32_t n, const uint32_t* off, const void* len, double delta, uint32_t* lend,
                             hipStream_t stream);
-hipError_t launch_reduce_slots(const double* bc_slot, uint32_t slots, uint32_t n, uint32_t stride, double* out, hipStream_t stream);
+hipError_t launch_reduce_slots(const BcRec* rec, uint32_t slots, uint32_t n, uint32_t stride, double* out, hipStream_t stream);
+// Sets every record to the clean state (all zero).
+hipError_t launch_rec_init(BcRec* rec, uint64_t count, hipStream_t stream);
+// Zeroes the betweenness partial of every record.
+hipError_t launch_rec_zero_bc(BcRec* rec, uint64_t count, hipStream_t stream);
+hipError_t launch_fill_dist(DKind dk, void* p, uint64_t count, hipStream_t stream);  // fills with +infinity / max
 // out[node_prefix[g] + v] = sum over k in [seg_ptr[g], seg_ptr[g+1]) of flush[seg_off[k] + v] (slot order).
 hipError_t launch_cohort_reduce(const double* flush, const uint32_t* node_prefix, uint32_t n_graphs, const uint32_t* seg_ptr,
                                 const uint64_t* seg_off, uint32_t total_nodes, double* out, hipStream_t stream);
 hipError_t launch_fill_u64(uint64_t* p, uint64_t count, uint64_t value, hipStream_t stream);
 hipError_t launch_fill_u32(uint32_t* p, uint64_t count, uint32_t value, hipStream_t stream);
 hipError_t launch_fill_f64(double* p, uint64_t count, double value, hipStream_t stream);
-hipError_t launch_fill_dist(DKind dk, void* p, uint64_t count, hipStream_t stream);  // fills with +infinity / max
 hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t stream);  // bytes % 16 == 0
 // Resident workgroups per CU of the batched kernel for (dk, mode).
 int batch_blocks_per_cu(DKind dk, int mode, uint32_t n, bool sorted);

@@ -350,20 +350,17 @@ struct KArgs {
     B* dist;
     uint32_t* fstamp;
     uint32_t* sstamp;
-    uint32_t* pstamp;
+    BcRec* rec;
+    double* delta2;
     uint32_t* lists;
     void* pile;
     uint32_t pcap;
     uint32_t* counters;
-    double* sigma;
-    double* delta1;
-    double* delta2;
-    uint32_t* indeg;
     uint32_t* order;
     uint32_t* layers;
     uint32_t* bits;
     uint64_t words;
-    uint2* log;
+    LogEntry* log;
     uint32_t logcap;
     uint32_t stride;    // nodes per slot in the workspace layout (>= n)
     // cohort launch (gtab != nullptr): graph table, item prefix per graph, item range per slot, flush layout
@@ -374,7 +371,6 @@ struct KArgs {
     const uint64_t* flush_base;
     uint32_t bm_words;  // words of the LDS bitmap of finalized nodes (0: no bitmap)
     uint32_t sorted;    // 1: a second LDS bitmap orders each bucket's settled nodes by id
-    double* bc_slot;
     double* grad1;
     double* grad2;
     ng_sssp_summary* summaries;
@@ -426,7 +422,7 @@ __device__ __forceinline__ uint32_t range_mask(uint32_t wi, uint32_t b, uint32_t
 #endif
 
 template <class D, int MODE>
-__global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void batch_kernel(const KArgs<D> a) {
     using T = DT<D>;
     using B = typename T::B;
     using L = typename T::L;
@@ -457,10 +453,12 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
     const uint32_t* lend = a.lend;
     const uint32_t* perm = a.perm;
     uint32_t bmw = a.bm_words;  // bitmap words of the current graph
-    B* dist = a.dist + so;
-    uint32_t* fstamp = a.fstamp + so;
+    B* const dist = a.dist + so;
+    auto DP = [dist](uint32_t v) -> B* { return dist + v; };
+    uint32_t* const fstamp = a.fstamp + so;
     uint32_t* sstamp = a.sstamp + so;
-    uint32_t* pstamp = a.pstamp + so;
+    BcRec* const R = (MODE != MODE_SSSP) ? a.rec + so : nullptr;
+    double* const delta2 = ((MODE & MODE_ARC2) != 0) ? a.delta2 + so : nullptr;
     uint32_t* lists = a.lists + so * 3;
     uint32_t* const Fa = lists;
     uint32_t* const Fb = lists + stride;
@@ -468,25 +466,15 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
     const uint32_t pcap = a.pcap;
     PileEntry<B>* const Pa = static_cast<PileEntry<B>*>(a.pile) + static_cast<size_t>(slot) * 2 * pcap;
     PileEntry<B>* const Pb = Pa + pcap;
-    double* sigma = nullptr;
-    double* delta1 = nullptr;
-    double* delta2 = nullptr;
-    uint32_t* indeg = nullptr;
     uint32_t* order = nullptr;
     uint32_t* layers = nullptr;
     uint32_t* bits = nullptr;
-    uint2* log = nullptr;
-    double* bc_slot = nullptr;
+    LogEntry* log = nullptr;
     if constexpr (MODE != MODE_SSSP) {
-        sigma = a.sigma + so;
-        delta1 = a.delta1 + so;
-        indeg = a.indeg + so;
         order = a.order + so;
         layers = a.layers + static_cast<size_t>(slot) * (stride + 1);
         bits = a.bits + static_cast<size_t>(slot) * a.words;
         log = a.log + static_cast<size_t>(slot) * a.logcap;
-        if constexpr ((MODE & MODE_ARC2) != 0) delta2 = a.delta2 + so;
-        if constexpr ((MODE & MODE_BC) != 0) bc_slot = a.bc_slot + so;
     }
     uint32_t* ctr = a.counters + static_cast<size_t>(slot) * 4;
     uint32_t phase = ctr[0], gen = ctr[1], pgen = ctr[2];
@@ -506,8 +494,8 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
     auto flush_bc = [&]() {
         if constexpr ((MODE & MODE_BC) != 0) {
             for (uint32_t v = tid; v < n; v += kBlock) {
-                a.flush[fpos + v] = bc_slot[v];
-                bc_slot[v] = 0.0;
+                a.flush[fpos + v] = R[v].bc;
+                R[v].bc = 0.0;
             }
             fpos += n;
         }
@@ -542,7 +530,6 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
             for (uint32_t v = tid; v < stride; v += kBlock) {
                 stc(&fstamp[v], 0u);
                 stc(&sstamp[v], 0u);
-                stc(&pstamp[v], 0u);
             }
             phase = gen = pgen = 1;
             bsync();
@@ -557,7 +544,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
 #pragma unroll
             for (int k = 0; k < 8; ++k) S.cnt[k] = 0;
             S.runmin = T::kInf;
-            stc(&dist[s], B(0));
+            stc(DP(s), B(0));
             Fa[0] = s;
         }
         uint32_t* Fc = Fa;
@@ -587,7 +574,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
         // distance probe of arc target t: skipped (fin) for nodes finalized in earlier buckets
         auto probe = [&](uint32_t t, L l, bool& fin, B& cur) {
             fin = use_bm && skip_ok && !(l < delta) && ((bm[t >> 5] >> (t & 31)) & 1u);
-            cur = fin ? B(0) : ldc(&dist[t]);
+            cur = fin ? B(0) : ldc(DP(t));
             c_probe += fin ? 0u : 1u;
         };
         // relaxation of arc e = (v, t) with length l from v at distance dv, given the probe (fin, cur); new
@@ -597,17 +584,18 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
             const B nd = T::add(dv, l);
             bool pf = false, pp = false;
             if (ok && !final_t && nd < cur) {
-                const B old = wg_min(&dist[t], nd);
+                const B old = wg_min(DP(t), nd);
                 if (nd < old) {
                     ++c_improve;
                     if (nd < hi) {
                         pf = wg_exch(&fstamp[t], nph) != nph;
                     } else {
                         atomicMin(&S.runmin, nd);
-                        // first parking of t in this source always fits (the pile keeps n entries in reserve);
-                        // a re-parking with the improved value only while the reserve is untouched, else the
-                        // next bucket advance reloads the distances (exact mode)
-                        if (wg_exch(&pstamp[t], gen) != gen) {
+                        // first parking of t in this source (its first reach: every later reach finds a finite
+                        // distance) always fits, the pile keeps n entries in reserve; a re-parking with the
+                        // improved value only while the reserve is untouched, else the next bucket advance
+                        // reloads the distances (exact mode)
+                        if (old == T::kInf) {
                             pp = true;
                         } else if (S.cnt[1] + kBlock < pcap - n) {
                             pp = true;
@@ -629,7 +617,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                     uint32_t base = 0;
                     if (lane == leader) base = atomicAdd(&S.cnt[3], static_cast<uint32_t>(__popcll(mask)));
                     base = __shfl(base, static_cast<int>(leader), 64);
-                    if (lg && base + rank < a.logcap) log[base + rank] = make_uint2(v, e);
+                    if (lg && base + rank < a.logcap) log[base + rank] = LogEntry{t, e, static_cast<unsigned long long>(nd)};
                 }
             }
         };
@@ -648,7 +636,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                 expand_relax<NG_UNROLL>(
                     Fc, nF, S, tgt, len,
                     [&](uint32_t v, B& dv, uint32_t& beg, uint32_t& cnt) {
-                        dv = ldc(&dist[v]);
+                        dv = ldc(DP(v));
                         beg = off[v];
                         cnt = lend[v] - beg;
                         c_light += cnt;
@@ -718,7 +706,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                 expand_relax<NG_UNROLL>(
                     Sl + settle_from, nS - settle_from, S, tgt, len,
                     [&](uint32_t v, B& dv, uint32_t& beg, uint32_t& cnt) {
-                        dv = ldc(&dist[v]);
+                        dv = ldc(DP(v));
                         beg = (MODE != MODE_SSSP) ? off[v] : lend[v];
                         cnt = off[v + 1] - beg;
                         c_settle += cnt;
@@ -783,7 +771,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                     t = pe.node;
                     const bool settled = use_bm ? ((bm[t >> 5] >> (t & 31)) & 1u) != 0u : ldc(&sstamp[t]) == gen;
                     if (!settled) {
-                        d = exact ? ldc(&dist[t]) : pe.val;
+                        d = exact ? ldc(DP(t)) : pe.val;
                         if (d < hi) {
                             tf = wg_exch(&fstamp[t], nph) != nph;
                         } else {
@@ -821,7 +809,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
             double sd = 0.0, se = 0.0;
             for (uint32_t i = tid; i < nS; i += kBlock) {
                 const uint32_t v = Sl[i];
-                const B d = ldc(&dist[v]);
+                const B d = ldc(DP(v));
                 if constexpr (T::kInt) su += static_cast<unsigned long long>(d);
                 else sd += T::val(d);
                 if (d != B(0)) se += 1.0 / T::val(d);
@@ -880,18 +868,17 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
             // ---- tight arcs: verify the logged candidates against the final distances
             const uint32_t nlog = S.cnt[3];
             c_logged += nlog;
-            auto mark = [&](uint32_t u, uint32_t e) {
-                const uint32_t t = tgt[e];
-                const B c = T::add(ldc(&dist[u]), len[e]);
-                if (c == ldc(&dist[t])) {
+            // arc e into t is tight when its candidate c equals the final distance of t
+            auto mark = [&](uint32_t t, uint32_t e, B c) {
+                if (c == ldc(DP(t))) {
                     wg_or(&bits[e >> 5], 1u << (e & 31));
-                    wg_add(&indeg[t], 1u);
+                    wg_add(&R[t].indeg, 1u);
                 }
             };
             if (nlog <= a.logcap) {
                 for (uint32_t i = tid; i < nlog; i += kBlock) {
-                    const uint2 r = log[i];
-                    mark(r.x, r.y);
+                    const LogEntry r = log[i];
+                    mark(r.t, r.e, static_cast<B>(r.nd));
                 }
             } else {
                 if (tid == 0) atomicOr(a.err, ERR_LOG_OVERFLOW);
@@ -902,7 +889,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                         beg = off[v];
                         cnt = off[v + 1] - beg;
                     },
-                    [&](uint32_t v, B, uint32_t e) { mark(v, e); });
+                    [&](uint32_t v, B, uint32_t e) { mark(tgt[e], e, T::add(ldc(DP(v)), len[e])); });
             }
             bsync();
             NG_TOCK(t_verify, 4);
@@ -910,7 +897,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
             // ---- Kahn layers over the tight DAG: sigma and a topological order
             if (tid == 0) {
                 order[0] = s;
-                stc(&sigma[s], 1.0);
+                stc(&R[s].sigma, 1.0);
                 S.cnt[4] = 1;
             }
             bsync();
@@ -920,7 +907,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                 ++nl;
                 for (uint32_t i = lb + tid; i < le; i += kBlock) {
                     const uint32_t u = order[i];
-                    const double su = ldc(&sigma[u]);
+                    const double su = ldc(&R[u].sigma);
                     const uint32_t b = off[u], e_end = off[u + 1];
                     if (b < e_end) {
                         for (uint32_t wi = b >> 5; wi <= (e_end - 1) >> 5; ++wi) {
@@ -929,8 +916,8 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                                 const uint32_t e = (wi << 5) + static_cast<uint32_t>(__ffs(w) - 1);
                                 w &= w - 1;
                                 const uint32_t t = tgt[e];
-                                wg_add(&sigma[t], su);
-                                if (wg_sub(&indeg[t], 1u) == 1u) {
+                                wg_add(&R[t].sigma, su);
+                                if (wg_sub(&R[t].indeg, 1u) == 1u) {
                                     const uint32_t pos = atomicAdd(&S.cnt[4], 1u);
                                     if (pos < n) order[pos] = t;
                                 }
@@ -949,7 +936,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                 if (tid == 0) atomicOr(a.err, ERR_KAHN);
                 for (uint32_t i = tid; i < nS; i += kBlock) {
                     const uint32_t u = Sl[i];
-                    stc(&indeg[u], 0u);
+                    stc(&R[u].indeg, 0u);
                     const uint32_t b = off[u], e_end = off[u + 1];
                     if (b < e_end)
                         for (uint32_t wi = b >> 5; wi <= (e_end - 1) >> 5; ++wi) wg_and(&bits[wi], ~range_mask(wi, b, e_end));
@@ -963,7 +950,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                 const uint32_t l0 = layers[ly], l1 = layers[ly + 1];
                 for (uint32_t i = l0 + tid; i < l1; i += kBlock) {
                     const uint32_t u = order[i];
-                    const double su = ldc(&sigma[u]);
+                    const double su = ldc(&R[u].sigma);
                     double acc1 = 0.0, acc2 = 0.0;
                     const uint32_t b = off[u], e_end = off[u + 1];
                     if (b < e_end) {
@@ -975,12 +962,12 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                                 const uint32_t e = (wi << 5) + static_cast<uint32_t>(__ffs(w) - 1);
                                 w &= w - 1;
                                 const uint32_t t = tgt[e];
-                                const double st = ldc(&sigma[t]);
-                                const double term1 = su * ((1.0 + ldc(&delta1[t])) / st);
+                                const double st = ldc(&R[t].sigma);
+                                const double term1 = su * ((1.0 + ldc(&R[t].delta1)) / st);
                                 acc1 += term1;
                                 if constexpr ((MODE & MODE_ARC1) != 0) unsafeAtomicAdd(&a.grad1[perm[e]], term1);
                                 if constexpr ((MODE & MODE_ARC2) != 0) {
-                                    const double dt = T::val(ldc(&dist[t]));
+                                    const double dt = T::val(ldc(DP(t)));
                                     const double term2 = su * ((1.0 / (dt * dt) + ldc(&delta2[t])) / st);
                                     acc2 += term2;
                                     unsafeAtomicAdd(&a.grad2[perm[e]], term2);
@@ -989,9 +976,9 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
                         }
                     }
                     if (ly > 0) {
-                        stc(&delta1[u], acc1);
+                        stc(&R[u].delta1, acc1);
                         if constexpr ((MODE & MODE_ARC2) != 0) stc(&delta2[u], acc2);
-                        if constexpr ((MODE & MODE_BC) != 0) bc_slot[u] += acc1;
+                        if constexpr ((MODE & MODE_BC) != 0) R[u].bc += acc1;
                     }
                 }
                 bsync();
@@ -1002,20 +989,20 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const KArgs<D> a) {
         NG_TICK(t_reset);
         if (a.dist_out != nullptr) {
             for (uint32_t v = tid; v < n; v += kBlock)
-                write_dist_out(a, static_cast<size_t>(si) * n + v, ldc(&dist[v]));
+                write_dist_out(a, static_cast<size_t>(si) * n + v, ldc(DP(v)));
             bsync();  // every lane has read its distances before any lane resets them below
         }
         // ---- restore the slot: only reached nodes were touched (after an abort, frontier and pile nodes
         // may hold distances too, so the whole array is reset)
         if (aborted) {
-            for (uint32_t v = tid; v < n; v += kBlock) stc(&dist[v], T::kInf);
+            for (uint32_t v = tid; v < n; v += kBlock) stc(DP(v), T::kInf);
         }
         for (uint32_t i = tid; i < nS; i += kBlock) {
             const uint32_t v = Sl[i];
-            stc(&dist[v], T::kInf);
+            stc(DP(v), T::kInf);
             if constexpr (MODE != MODE_SSSP) {
-                stc(&sigma[v], 0.0);
-                stc(&delta1[v], 0.0);
+                stc(&R[v].sigma, 0.0);
+                stc(&R[v].delta1, 0.0);
                 if constexpr ((MODE & MODE_ARC2) != 0) stc(&delta2[v], 0.0);
             }
         }
@@ -1059,13 +1046,23 @@ __global__ void light_end_kernel(uint32_t n, const uint32_t* off, const L* len, 
     lend[v] = lo;
 }
 
-// out[v] = sum over slots of part[slot * stride + v], in slot order (deterministic)
-__global__ void reduce_slots_kernel(const double* part, uint32_t slots, uint32_t n, uint32_t stride, double* out) {
+// out[v] = sum over slots of the betweenness partial of node v, in slot order (deterministic)
+__global__ void reduce_slots_kernel(const BcRec* rec, uint32_t slots, uint32_t n, uint32_t stride, double* out) {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
     double acc = 0.0;
-    for (uint32_t k = 0; k < slots; ++k) acc += part[static_cast<size_t>(k) * stride + v];
+    for (uint32_t k = 0; k < slots; ++k) acc += rec[static_cast<size_t>(k) * stride + v].bc;
     out[v] = acc;
+}
+
+__global__ void rec_init_kernel(BcRec* rec, uint64_t count) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) rec[i] = BcRec{};
+}
+
+__global__ void rec_zero_bc_kernel(BcRec* rec, uint64_t count) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) rec[i].bc = 0.0;
 }
 
 template <class T>
@@ -1124,22 +1121,18 @@ hipError_t launch_typed(const LaunchArgs& la, hipStream_t stream) {
     k.dist = static_cast<B*>(ws.dist);
     k.fstamp = ws.fstamp;
     k.sstamp = ws.sstamp;
-    k.pstamp = ws.pstamp;
+    k.rec = ws.rec;
+    k.delta2 = ws.delta2;
     k.lists = ws.lists;
     k.pile = ws.pile;
     k.pcap = ws.pcap;
     k.counters = ws.counters;
-    k.sigma = ws.sigma;
-    k.delta1 = ws.delta1;
-    k.delta2 = ws.delta2;
-    k.indeg = ws.indeg;
     k.order = ws.order;
     k.layers = ws.layers;
     k.bits = ws.bits;
     k.words = ws.words;
     k.log = ws.log;
     k.logcap = ws.logcap;
-    k.bc_slot = ws.bc_slot;
     k.grad1 = ws.grad1;
     k.grad2 = ws.grad2;
     k.summaries = static_cast<ng_sssp_summary*>(ws.summaries);
@@ -1256,9 +1249,9 @@ hipError_t launch_cohort_reduce(const double* flush, const uint32_t* node_prefix
     return hipGetLastError();
 }
 
-hipError_t launch_reduce_slots(const double* bc_slot, uint32_t slots, uint32_t n, uint32_t stride, double* out, hipStream_t stream) {
+hipError_t launch_reduce_slots(const BcRec* rec, uint32_t slots, uint32_t n, uint32_t stride, double* out, hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(reduce_slots_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, bc_slot, slots, n, stride, out);
+    hipLaunchKernelGGL(reduce_slots_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, rec, slots, n, stride, out);
     return hipGetLastError();
 }
 
@@ -1284,6 +1277,19 @@ hipError_t launch_fill_u64(uint64_t* p, uint64_t count, uint64_t value, hipStrea
 hipError_t launch_fill_f64(double* p, uint64_t count, double value, hipStream_t stream) {
     return fill<double>(p, count, value, stream);
 }
+namespace {
+uint64_t grid_for(uint64_t count) {
+    const uint64_t b = (count + 255) / 256;
+    return b < 65536 ? (b ? b : 1) : 65536;
+}
+}  // namespace
+
+hipError_t launch_rec_init(BcRec* rec, uint64_t count, hipStream_t stream) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(rec_init_kernel, dim3(static_cast<uint32_t>(grid_for(count))), dim3(256), 0, stream, rec, count);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_dist(DKind dk, void* p, uint64_t count, hipStream_t stream) {
     switch (dk) {
     case DKind::U32: return fill<uint32_t>(static_cast<uint32_t*>(p), count, 0xFFFFFFFFu, stream);
@@ -1292,6 +1298,12 @@ hipError_t launch_fill_dist(DKind dk, void* p, uint64_t count, hipStream_t strea
     case DKind::F64: return fill<unsigned long long>(static_cast<unsigned long long*>(p), count, 0x7ff0000000000000ull, stream);
     }
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_rec_zero_bc(BcRec* rec, uint64_t count, hipStream_t stream) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(rec_zero_bc_kernel, dim3(static_cast<uint32_t>(grid_for(count))), dim3(256), 0, stream, rec, count);
+    return hipGetLastError();
 }
 
 }  // namespace ng
