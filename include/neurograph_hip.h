@@ -120,6 +120,15 @@ ng_status ng_graph_dist_kind(const ng_graph* g, ng_length_kind lkind, ng_dist_ki
 ng_status ng_sssp_summaries(ng_graph* g, const uint32_t* sources, uint32_t n_sources, ng_length_kind lkind,
                             double delta, ng_sssp_summary* out, void* dist_out);
 
+/* Histogram of the finite shortest-path distances d(s, t), t != s, over `sources` (lengths lkind, as in
+ * ng_sssp_summaries): counts[i] = #{d : i <= d / bin_width < i + 1} for i < n_bins - 1; the last bin takes
+ * everything beyond. Built on the device from the distance rows of the batched pass, in source chunks of at
+ * most 1 GiB of rows; out_opt[n_sources] (optional) receives the per-source summaries of the same pass.
+ * The reach-size histogram of the reference's output is ApspGlobalResult::cluster_sizes, rebuilt from the
+ * summaries (AllPairsShortestPath.h:140-141); this one is new (no reference counterpart). */
+ng_status ng_distance_histogram(ng_graph* g, const uint32_t* sources, uint32_t n_sources, ng_length_kind lkind, double delta,
+                                double bin_width, uint32_t n_bins, uint64_t* counts, ng_sssp_summary* out_opt);
+
 /* Brandes betweenness over `sources` with lengths |w|, accumulated into bc_inout[n] like
  * local_betweenness in BrandesVariant::accumulate_betweenness (BrandesVariant.h:50-74, :157) and the
  * dispatcher BetweennessCentrality::compute_average_betweenness_centrality (BetweennessCentrality.h:46-80).

@@ -79,6 +79,7 @@ def lib() -> ctypes.CDLL:
         "ng_graph_dist_kind": (i32, [vp, i32, P(i32)]),
         "ng_sssp_summaries": (i32, [vp, vp, u32, i32, dbl, vp, vp]),
         "ng_brandes_accumulate": (i32, [vp, vp, u32, dbl, vp, vp]),
+        "ng_distance_histogram": (i32, [vp, vp, u32, i32, dbl, dbl, u32, vp, vp]),
         "ng_edge_dependency": (i32, [vp, vp, u32, dbl, vp]),
         "ng_apsp_adjoint": (i32, [vp, vp, u32, dbl, vp, vp, vp]),
         "ng_autotune_delta": (i32, [vp, i32, i32, vp, u32, P(dbl)]),
@@ -270,6 +271,16 @@ class Graph:
         dist = np.zeros((len(src), self.n), dtype=self.dist_dtype(lkind)) if want_dist else None
         _check(lib().ng_sssp_summaries(self.h, _ptr(src), len(src), lkind, float(delta), _ptr(out), _ptr(dist)))
         return (out, dist) if want_dist else out
+
+    def distance_histogram(self, bin_width: float, n_bins: int, sources=None, lkind: int = NG_LEN_ABS, delta: float = 0.0,
+                           summaries: bool = False):
+        """Counts of finite d(s, t), t != s, in bins [i*w, (i+1)*w); the last bin takes everything beyond."""
+        src = self._sources(sources)
+        counts = np.zeros(n_bins, dtype=np.uint64)
+        out = np.zeros(len(src), dtype=SUMMARY_DTYPE) if summaries else None
+        _check(lib().ng_distance_histogram(self.h, _ptr(src), len(src), lkind, float(delta), float(bin_width), n_bins,
+                                           _ptr(counts), _ptr(out)))
+        return (counts, out) if summaries else counts
 
     def brandes(self, sources=None, delta: float = 0.0, bc=None, summaries: bool = False):
         src = self._sources(sources)

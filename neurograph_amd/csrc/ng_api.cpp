@@ -433,6 +433,7 @@ struct RunSpec {
     const uint32_t* sources;
     uint32_t n_sources;
     void* dist_out;   // host, optional
+    void* dist_dev = nullptr;  // device, optional (n_sources * n elements, kept on the device)
 };
 
 // Uploads sources, sizes the workspace, launches the batched kernel once, and synchronizes.
@@ -483,7 +484,7 @@ ng_status run_batch(ng_graph* g, const RunSpec& rs, uint32_t* slots_used, double
     if (rs.mode & MODE_BC) NG_HIP(launch_rec_zero_bc(w.rec, static_cast<uint64_t>(slots) * w.n, s));
     if (rs.mode & MODE_ARC1) NG_HIP(launch_fill_f64(w.grad1, g->m, 0.0, s));
     if (rs.mode & MODE_ARC2) NG_HIP(launch_fill_f64(w.grad2, g->m, 0.0, s));
-    void* d_dist_out = nullptr;
+    void* d_dist_out = rs.dist_dev;
     const int dok = dist_kind_of(v.dk);
     const size_t dob = (dok == NG_DIST_F32 ? 4 : 8);
     if (rs.dist_out) NG_HIP(hipMalloc(&d_dist_out, dob * std::max<size_t>(static_cast<size_t>(rs.n_sources) * g->n, 1)));
@@ -505,14 +506,14 @@ ng_status run_batch(ng_graph* g, const RunSpec& rs, uint32_t* slots_used, double
         NG_HIP(hipEventRecord(ctx->ev0, s));
         hipError_t le = launch_batch(la, s);
         if (le != hipSuccess) {
-            if (d_dist_out) (void)hipFree(d_dist_out);
+            if (rs.dist_out) (void)hipFree(d_dist_out);
             return fail(NG_EDEVICE, "batched kernel launch failed: %s", hipGetErrorString(le));
         }
         NG_HIP(hipEventRecord(ctx->ev1, s));
     }
     hipError_t se = hipStreamSynchronize(s);
     if (se != hipSuccess) {
-        if (d_dist_out) (void)hipFree(d_dist_out);
+        if (rs.dist_out) (void)hipFree(d_dist_out);
         return fail(NG_EDEVICE, "batched kernel failed: %s", hipGetErrorString(se));
     }
     if (rs.n_sources) NG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -729,6 +730,70 @@ ng_status ng_sssp_summaries(ng_graph* g, const uint32_t* sources, uint32_t n_sou
     ng_status st = run_batch(g, rs, nullptr, nullptr);
     if (st != NG_OK) return st;
     return copy_summaries(g, n_sources, out);
+}
+
+ng_status ng_distance_histogram(ng_graph* g, const uint32_t* sources, uint32_t n_sources, ng_length_kind lkind, double delta,
+                                double bin_width, uint32_t n_bins, uint64_t* counts, ng_sssp_summary* out_opt) {
+    if (check_graph(g) != NG_OK) return NG_EINVAL;
+    if (n_sources && !sources) return fail(NG_EINVAL, "null sources");
+    if (!counts || n_bins == 0) return fail(NG_EINVAL, "empty histogram");
+    if (!(bin_width > 0.0) || !std::isfinite(bin_width)) return fail(NG_EINVAL, "bin width must be positive and finite");
+    int vi = 0;
+    switch (lkind) {
+    case NG_LEN_ABS: vi = 0; break;
+    case NG_LEN_INVERSE: vi = 1; break;
+    case NG_LEN_RAW:
+        if (g->has_negative) return fail(NG_EINVAL, "negative arc weight: delta-stepping casts weights to distance_type (PushCheckVariant.h:82)");
+        vi = 0;
+        break;
+    default: return fail(NG_EINVAL, "unknown length kind %d", lkind);
+    }
+    ng_status st = ensure_view(g, vi);
+    if (st != NG_OK) return st;
+    NG_HIP(hipSetDevice(g->ctx->device));
+    const int dok = dist_kind_of(g->views[vi].dk);
+    const size_t dob = dok == NG_DIST_F32 ? 4 : 8;
+    // sources in chunks whose distance rows take at most 1 GiB of device memory
+    const uint64_t row = std::max<uint64_t>(static_cast<uint64_t>(g->n) * dob, 1);
+    const uint32_t chunk = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(n_sources, (1ull << 30) / row)));
+    void* d_dist = nullptr;
+    unsigned long long* d_counts = nullptr;
+    uint32_t* d_src = nullptr;
+    auto cleanup = [&]() {
+        if (d_dist) (void)hipFree(d_dist);
+        if (d_counts) (void)hipFree(d_counts);
+        if (d_src) (void)hipFree(d_src);
+    };
+    if (hipMalloc(&d_dist, row * std::max<uint32_t>(chunk, 1)) != hipSuccess || hipMalloc(&d_counts, 8ull * n_bins) != hipSuccess ||
+        hipMalloc(&d_src, 4ull * std::max<uint32_t>(chunk, 1)) != hipSuccess) {
+        cleanup();
+        return fail(NG_ENOMEM, "histogram buffers");
+    }
+    hipError_t he = launch_fill_u64(reinterpret_cast<uint64_t*>(d_counts), n_bins, 0, g->ctx->stream);
+    for (uint32_t b = 0; he == hipSuccess && b < n_sources; b += chunk) {
+        const uint32_t k = std::min(chunk, n_sources - b);
+        RunSpec rs{vi, MODE_SSSP, delta, sources + b, k, nullptr};
+        rs.dist_dev = d_dist;
+        st = run_batch(g, rs, nullptr, nullptr);
+        if (st != NG_OK) {
+            cleanup();
+            return st;
+        }
+        if (out_opt) {
+            st = copy_summaries(g, k, out_opt + b);
+            if (st != NG_OK) {
+                cleanup();
+                return st;
+            }
+        }
+        he = hipMemcpyAsync(d_src, sources + b, 4ull * k, hipMemcpyHostToDevice, g->ctx->stream);
+        if (he == hipSuccess) he = launch_distance_histogram(dok, d_dist, d_src, k, g->n, bin_width, n_bins, d_counts, g->ctx->stream);
+    }
+    if (he == hipSuccess) he = hipStreamSynchronize(g->ctx->stream);
+    if (he == hipSuccess) he = hipMemcpy(counts, d_counts, 8ull * n_bins, hipMemcpyDeviceToHost);
+    cleanup();
+    if (he != hipSuccess) return fail(NG_EDEVICE, "distance histogram failed: %s", hipGetErrorString(he));
+    return NG_OK;
 }
 
 ng_status ng_brandes_accumulate(ng_graph* g, const uint32_t* sources, uint32_t n_sources, double delta, double* bc_inout,

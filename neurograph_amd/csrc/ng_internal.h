@@ -148,6 +148,10 @@ hipError_t launch_fill_dist(DKind dk, void* p, uint64_t count, hipStream_t strea
 hipError_t launch_cohort_reduce(const double* flush, const uint32_t* node_prefix, uint32_t n_graphs, const uint32_t* seg_ptr,
                                 const uint64_t* seg_off, uint32_t total_nodes, double* out, hipStream_t stream);
 hipError_t launch_fill_u64(uint64_t* p, uint64_t count, uint64_t value, hipStream_t stream);
+// counts[min(floor(d / bin_width), n_bins - 1)] += 1 for every finite d = dist[i * n + t], t != sources[i] (dist_kind:
+// ng_dist_kind of the rows; unreachable = the kind's max value).
+hipError_t launch_distance_histogram(int dist_kind, const void* dist, const uint32_t* sources, uint32_t n_rows, uint32_t n,
+                                     double bin_width, uint32_t n_bins, unsigned long long* counts, hipStream_t stream);
 hipError_t launch_fill_u32(uint32_t* p, uint64_t count, uint32_t value, hipStream_t stream);
 hipError_t launch_fill_f64(double* p, uint64_t count, double value, hipStream_t stream);
 hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t stream);  // bytes % 16 == 0

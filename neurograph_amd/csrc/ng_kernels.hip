@@ -29,6 +29,7 @@
 
 #include <cfloat>
 #include <cstdint>
+#include <type_traits>
 
 namespace ng {
 namespace {
@@ -266,7 +267,7 @@ __device__ __forceinline__ void expand(const uint32_t* list, uint32_t count, Lds
 // bound because a lane's arc index only grows. apply() runs on every active lane (it may append to
 // queues with wave ballots), with ok = false for padding slots.
 #ifndef NG_UNROLL
-#define NG_UNROLL 4
+#define NG_UNROLL 2
 #endif
 template <int U, class B, class L, class NodeFn, class ProbeFn, class ApplyFn>
 __device__ __forceinline__ void expand_relax(const uint32_t* list, uint32_t count, Lds<B>& S, const uint32_t* tgt,
@@ -422,7 +423,13 @@ __device__ __forceinline__ uint32_t range_mask(uint32_t wi, uint32_t b, uint32_t
 #endif
 
 template <class D, int MODE>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void batch_kernel(const KArgs<D> a) {
+// Resident waves per SIMD the hot kernel is compiled for (the register budget follows: 512 / waves VGPRs).
+// The path is bound by the latency of random accesses, so occupancy matters more than the few registers
+// spilled to fit.
+#ifndef NG_WAVES_PER_EU
+#define NG_WAVES_PER_EU 5
+#endif
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NG_WAVES_PER_EU))) void batch_kernel(const KArgs<D> a) {
     using T = DT<D>;
     using B = typename T::B;
     using L = typename T::L;
@@ -1240,6 +1247,76 @@ __global__ void cohort_reduce_kernel(const double* flush, const uint32_t* node_p
     out[i] = acc;
 }
 }  // namespace
+
+namespace {
+// Block-privatised histogram of one tile of distance rows: LDS bins for up to kHistLds bins, global atomics
+// beyond. Row i (source sources[i]) holds n distances; the source itself and unreachable nodes are skipped.
+constexpr uint32_t kHistLds = 4096;
+
+template <class T>
+__device__ __forceinline__ bool hist_bin(T d, double bw, uint32_t nb, uint32_t* bin) {
+    double x;
+    if constexpr (sizeof(T) == 8 && !std::is_floating_point_v<T>) {
+        if (d == ~0ull) return false;
+        x = static_cast<double>(d);
+    } else if constexpr (std::is_same_v<T, float>) {
+        if (d == FLT_MAX) return false;
+        x = static_cast<double>(d);
+    } else {
+        if (d == DBL_MAX) return false;
+        x = d;
+    }
+    const double q = floor(x / bw);
+    *bin = q >= static_cast<double>(nb - 1) ? nb - 1 : static_cast<uint32_t>(q);
+    return true;
+}
+
+template <class T>
+__global__ void distance_histogram_kernel(const T* dist, const uint32_t* sources, uint32_t n_rows, uint32_t n, double bw,
+                                          uint32_t nb, unsigned long long* counts) {
+    __shared__ uint32_t lds[kHistLds];
+    const bool local = nb <= kHistLds;
+    if (local)
+        for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) lds[i] = 0u;
+    __syncthreads();
+    const uint64_t total = static_cast<uint64_t>(n_rows) * n;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const uint32_t r = static_cast<uint32_t>(i / n), t = static_cast<uint32_t>(i - static_cast<uint64_t>(r) * n);
+        if (t == sources[r]) continue;
+        uint32_t bin;
+        if (!hist_bin<T>(dist[i], bw, nb, &bin)) continue;
+        if (local) atomicAdd(&lds[bin], 1u);
+        else atomicAdd(&counts[bin], 1ull);
+    }
+    __syncthreads();
+    if (local)
+        for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x)
+            if (lds[i]) atomicAdd(&counts[i], static_cast<unsigned long long>(lds[i]));
+}
+}  // namespace
+
+hipError_t launch_distance_histogram(int dist_kind, const void* dist, const uint32_t* sources, uint32_t n_rows, uint32_t n,
+                                     double bin_width, uint32_t n_bins, unsigned long long* counts, hipStream_t stream) {
+    const uint64_t total = static_cast<uint64_t>(n_rows) * n;
+    if (total == 0) return hipSuccess;
+    const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((total + 255) / 256, 4096));
+    switch (dist_kind) {
+    case NG_DIST_U64:
+        hipLaunchKernelGGL(distance_histogram_kernel<unsigned long long>, dim3(blocks), dim3(256), 0, stream,
+                           static_cast<const unsigned long long*>(dist), sources, n_rows, n, bin_width, n_bins, counts);
+        break;
+    case NG_DIST_F32:
+        hipLaunchKernelGGL(distance_histogram_kernel<float>, dim3(blocks), dim3(256), 0, stream, static_cast<const float*>(dist),
+                           sources, n_rows, n, bin_width, n_bins, counts);
+        break;
+    default:
+        hipLaunchKernelGGL(distance_histogram_kernel<double>, dim3(blocks), dim3(256), 0, stream, static_cast<const double*>(dist),
+                           sources, n_rows, n, bin_width, n_bins, counts);
+        break;
+    }
+    return hipGetLastError();
+}
 
 hipError_t launch_cohort_reduce(const double* flush, const uint32_t* node_prefix, uint32_t n_graphs, const uint32_t* seg_ptr,
                                 const uint64_t* seg_off, uint32_t total_nodes, double* out, hipStream_t stream) {

@@ -6,11 +6,15 @@
 //          [--apsp-delta D] [--apsp-delta-epoch push-check|...|gpu] [--dijkstra-queue priority-queue|...|gpu]
 //          [--centrality-algorithm brandes|path-enumeration|gpu] [--undirected] [--all-weights-one]
 //          [--remove-autapses] [--prefix P] [--output-directory D] [--threads N]
-//          [--weights int|f32|f64] [--device N] [--node-betweenness]
+//          [--weights int|f32|f64] [--device N] [--node-betweenness] [--distance-matrix]
+//          [--distance-histogram-width W] [--distance-histogram-bins B]
 //
 // Every selector value runs the GPU variant (they compute identical values); the reference's refusal of
 // the bucket queue with avg-apsp-inv is kept (neurograph.cpp:690-693). --weights f32|f64 reads real-valued
-// weights (extension); --node-betweenness also writes per-node values (extension).
+// weights (extension); --node-betweenness also writes per-node values (extension). With avg-apsp,
+// --distance-matrix writes all_pairs_distances.bin (row s = distances from node s, little-endian u64 for integer
+// lengths, f32 for f32 weights, f64 otherwise; unreachable = the type's max) and --distance-histogram-width W
+// writes all_pairs_distance_histogram.txt in the reference's histogram text format (MetricStore.cpp:32-40).
 
 #include <algorithm>
 #include <charconv>
@@ -19,7 +23,9 @@
 #include <filesystem>
 #include <fstream>
 #include <iostream>
+#include <limits>
 #include <map>
+#include <numeric>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -73,6 +79,7 @@ int main(int argc, char** argv) {
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
         if (a == "--undirected" || a == "--all-weights-one" || a == "--remove-autapses" || a == "--node-betweenness" ||
+            a == "--distance-matrix" ||
             a == "--disable-status-report" || a == "--disable-timer-report") {
             flags.push_back(a);
         } else if (a.rfind("--", 0) == 0 && i + 1 < argc) {
@@ -144,6 +151,47 @@ int main(int argc, char** argv) {
             } else {
                 emit(out_dir, "all_pairs_shortest_paths.txt", apsp_text(ng_host::AllPairsShortestPath::compute_apsp_real(graph)));
             }
+        }
+        if (apsp && has("--distance-matrix")) {
+            // rows in chunks of at most 256 MiB
+            const ng_length_kind lk = algo == "delta-stepping" ? NG_LEN_RAW : NG_LEN_ABS;
+            const double dd = algo == "delta-stepping" ? std::stod(get("--apsp-delta", "1")) : 0.0;
+            ng_dist_kind dkind{};
+            ng_host::check(ng_graph_dist_kind(graph.get(), lk, &dkind), "ng_graph_dist_kind");
+            const size_t eb = dkind == NG_DIST_F32 ? 4 : 8;
+            const uint32_t n = graph.number_nodes();
+            const uint32_t chunk = static_cast<uint32_t>(std::max<size_t>(1, (256ull << 20) / (eb * std::max<uint32_t>(n, 1))));
+            std::ofstream f(out_dir / "all_pairs_distances.bin", std::ios::binary);
+            std::vector<char> rows(static_cast<size_t>(chunk) * n * eb);
+            std::vector<ng_sssp_summary> summ(chunk);
+            for (uint32_t b = 0; b < n; b += chunk) {
+                const uint32_t k = std::min(chunk, n - b);
+                std::vector<uint32_t> src(k);
+                std::iota(src.begin(), src.end(), b);
+                ng_host::check(ng_sssp_summaries(graph.get(), src.data(), k, lk, dd, summ.data(), rows.data()), "ng_sssp_summaries");
+                f.write(rows.data(), static_cast<std::streamsize>(static_cast<size_t>(k) * n * eb));
+            }
+            std::cout << "Wrote the " << n << " x " << n << " distance matrix ("
+                      << (dkind == NG_DIST_U64 ? "u64" : dkind == NG_DIST_F32 ? "f32" : "f64") << ")\n";
+        }
+        if (apsp && opt.count("--distance-histogram-width")) {
+            const ng_length_kind lk = algo == "delta-stepping" ? NG_LEN_RAW : NG_LEN_ABS;
+            const double dd = algo == "delta-stepping" ? std::stod(get("--apsp-delta", "1")) : 0.0;
+            const double width = std::stod(opt["--distance-histogram-width"]);
+            const uint32_t bins = static_cast<uint32_t>(std::stoul(get("--distance-histogram-bins", "256")));
+            std::vector<uint64_t> counts(bins);
+            const auto sources = graph.all_sources();
+            ng_host::check(ng_distance_histogram(graph.get(), sources.data(), static_cast<uint32_t>(sources.size()), lk, dd, width, bins,
+                                                 counts.data(), nullptr),
+                           "ng_distance_histogram");
+            std::ostringstream ss;
+            for (uint32_t i = 0; i < bins; ++i) {
+                ss << i << ". bin: " << num(width * i) << '-';
+                if (i + 1 < bins) ss << num(width * (i + 1));
+                else ss << num(std::numeric_limits<double>::max());
+                ss << ": " << counts[i] << '\n';
+            }
+            emit(out_dir, "all_pairs_distance_histogram.txt", ss.str());
         }
         if (apsp_inv) {
             emit(out_dir, "all_pairs_shortest_paths_inverse.txt", apsp_text(ng_host::AllPairsShortestPath::compute_apsp_inverse(graph)));

@@ -92,3 +92,26 @@ def test_cli_real_weights_and_node_output(tmp_path):
     assert clusters == ref["cluster_sizes"]
     per = np.loadtxt(os.path.join(d, "metrics", "node_betweenness_centrality.txt"))
     np.testing.assert_allclose(per[:, 1], orc.brandes(c, threads=8)["bc"], rtol=1e-9, atol=1e-9)
+
+
+def test_cli_distance_matrix_and_histogram(tmp_path):
+    """C1 outputs: the all-pairs distance matrix (u64 rows) and the distance histogram in the reference's
+    histogram text format."""
+    import neurograph_amd as ng
+
+    hg = ng.generate_fixed_degree(600, 6, 1, 10, 0)
+    d = str(tmp_path / "c1")
+    ng.write_graph_dir(d, hg)
+    r = run_cli("--input", d, "--func", "avg-apsp", "--distance-matrix", "--distance-histogram-width", "1",
+                "--distance-histogram-bins", "48")
+    assert r.returncode == 0, r.stderr
+    c = orc.Csr(hg.n, hg.offsets, hg.targets, hg.weights)
+    ref = orc.brandes(c, want_dist=True, threads=8)["dist"]
+    mat = np.fromfile(os.path.join(d, "metrics", "all_pairs_distances.bin"), dtype=np.uint64).reshape(hg.n, hg.n)
+    np.testing.assert_array_equal(mat, ref)
+    lines = open(os.path.join(d, "metrics", "all_pairs_distance_histogram.txt")).read().splitlines()
+    assert len(lines) == 48 and lines[3].startswith("3. bin: 3-4: ")
+    counts = np.array([int(x.rsplit(": ", 1)[1]) for x in lines])
+    offdiag = ref[~np.eye(hg.n, dtype=bool)]
+    fin = offdiag[offdiag != np.iinfo(np.uint64).max]
+    np.testing.assert_array_equal(counts, np.bincount(np.minimum(fin, 47).astype(np.int64), minlength=48))

@@ -280,3 +280,31 @@ def test_shared_workspace_across_graphs(ctx):
         _, dist = g.sssp(srcs, want_dist=True)
         np.testing.assert_array_equal(dist, ref)
         np.testing.assert_allclose(g.brandes(srcs), orc.brandes(c, sources=srcs)["bc"], rtol=1e-9, atol=1e-9)
+
+
+@pytest.mark.parametrize("wdtype", [np.int32, np.float32, np.float64])
+def test_distance_histogram_and_matrix(ctx, wdtype):
+    """The device distance histogram equals numpy's binning of the oracle's distance matrix, count for count;
+    the matrix itself is bit-exact (fixed-degree C1 generator for integer weights)."""
+    import neurograph_amd as ng
+
+    if wdtype == np.int32:
+        hg = ng.generate_fixed_degree(1500, 6, 1, 10, 0)
+        width, bins = 1.0, 64
+    else:
+        hg = ng.generate_spatial(1500, 12, weights=wdtype, seed=3)
+        width, bins = 0.25, 40
+    c = orc.Csr(hg.n, hg.offsets, hg.targets, hg.weights)
+    g = ng.Graph(ctx, hg)
+    srcs = np.arange(0, hg.n, 3)
+    ref = orc.brandes(c, sources=srcs, want_dist=True, threads=8)["dist"]
+    _, dist = g.sssp(srcs, want_dist=True)
+    np.testing.assert_array_equal(dist, ref)
+    counts, summ = g.distance_histogram(width, bins, srcs, summaries=True)
+    unreach = np.iinfo(np.uint64).max if ref.dtype == np.uint64 else np.finfo(ref.dtype).max
+    mask = ref != unreach
+    mask[np.arange(len(srcs)), srcs] = False
+    d = ref[mask].astype(np.float64)
+    b = np.minimum(np.floor(d / width), bins - 1).astype(np.int64)
+    np.testing.assert_array_equal(counts, np.bincount(b, minlength=bins).astype(np.uint64))
+    assert int(counts.sum()) == len(srcs) * (hg.n - 1) - int(summ["n_unreached"].sum())
