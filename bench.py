@@ -291,11 +291,15 @@ def main():
           f"{steps} steps", file=sys.stderr, flush=True)
 
     delta = args.delta
+    block = None
     if delta <= 0:
+        # bucket width and lanes per workgroup, tuned together on a sample large enough to fill the GPU
         rng = np.random.default_rng(1)
-        sample = np.sort(rng.choice(sources, size=min(len(sources), 512), replace=False))
-        delta = g.autotune(sample, mode=1)
-    print(f"[bench] delta {delta:.6g}", file=sys.stderr, flush=True)
+        sample = np.sort(rng.choice(sources, size=min(len(sources), 3072), replace=False))
+        t0 = time.perf_counter()
+        delta, block = g.autotune_shape(sample, mode=1)
+        print(f"[bench] autotune {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    print(f"[bench] delta {delta:.6g} block {block}", file=sys.stderr, flush=True)
 
     sync = ctx.synchronize  # every ng call synchronizes already; this brackets the timed region explicitly
 
@@ -347,6 +351,7 @@ def main():
         "dtype": {np.dtype(np.int32): "u32", np.dtype(np.float32): "f32", np.dtype(np.float64): "f64"}[hg.weights.dtype],
         "data": "synthetic",
         "config": {"workload": args.workload, **cfg, "arcs": hg.m, "sources_per_step": batch, "delta": delta,
+                   "lanes_per_source": g.last_stats()["block"],
                    "outputs": "per-source APSP summaries + betweenness in one pass"},
         "gteps": round(arcs / wall / 1e9, 3),
         "kernel_ms_per_step": round(sum(kernel_ms) / steps, 3),

@@ -65,7 +65,7 @@ typedef struct {
     uint64_t arcs_reached;   /* sum over sources of out-arcs of reached nodes (TEPS numerator) */
     uint32_t flags;          /* informational device flags of the last call (2: tight-arc log overflowed
                                 and the full-row scan was used; 4: in-bucket work after a settle pass) */
-    uint32_t reserved;
+    uint32_t block;          /* lanes per workgroup (one source slot) used */
     /* work counters of the last call, summed over sources */
     uint64_t phases;         /* frontier phases (light-arc rounds) */
     uint64_t buckets;        /* bucket advances */
@@ -156,6 +156,15 @@ ng_status ng_apsp_adjoint(ng_graph* g, const uint32_t* sources, uint32_t n_sourc
  * integer --apsp-delta (neurograph.cpp:515); tuning is this build's extension. */
 ng_status ng_autotune_delta(ng_graph* g, ng_length_kind lkind, int mode, const uint32_t* sample, uint32_t n_sample,
                             double* delta_out);
+
+/* Picks the bucket width and the lanes per workgroup (256, 512 or 1024) together for (lkind, mode) by timing
+ * the batched kernel on `sample` sources, and stores both in the graph (used whenever delta <= 0). Fewer,
+ * wider workgroups keep fewer sources in flight, so their per-source state stays cache-resident on large
+ * graphs. The sample should hold at least twice as many sources as the GPU runs at once with 256-lane
+ * workgroups (5 per CU), so that every candidate is timed at full occupancy. Extension, like
+ * ng_autotune_delta. */
+ng_status ng_autotune(ng_graph* g, ng_length_kind lkind, int mode, const uint32_t* sample, uint32_t n_sample, double* delta_out,
+                      uint32_t* block_out);
 
 /* Stats of the last successful call on `g`. */
 ng_status ng_graph_last_stats(const ng_graph* g, ng_call_stats* out);

@@ -44,7 +44,7 @@ assert SUMMARY_DTYPE.itemsize == ctypes.sizeof(Summary)
 class CallStats(ctypes.Structure):
     _fields_ = [("kernel_ms", ctypes.c_double), ("launches", ctypes.c_uint32), ("slots", ctypes.c_uint32),
                 ("delta", ctypes.c_double), ("arcs_reached", ctypes.c_uint64), ("flags", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32), ("phases", ctypes.c_uint64), ("buckets", ctypes.c_uint64),
+                ("block", ctypes.c_uint32), ("phases", ctypes.c_uint64), ("buckets", ctypes.c_uint64),
                 ("light_arcs", ctypes.c_uint64), ("settle_arcs", ctypes.c_uint64), ("logged_arcs", ctypes.c_uint64),
                 ("layers", ctypes.c_uint64), ("probes", ctypes.c_uint64), ("improvements", ctypes.c_uint64),
                 ("section_ticks", ctypes.c_uint64 * 8)]
@@ -83,6 +83,7 @@ def lib() -> ctypes.CDLL:
         "ng_edge_dependency": (i32, [vp, vp, u32, dbl, vp]),
         "ng_apsp_adjoint": (i32, [vp, vp, u32, dbl, vp, vp, vp]),
         "ng_autotune_delta": (i32, [vp, i32, i32, vp, u32, P(dbl)]),
+        "ng_autotune": (i32, [vp, i32, i32, vp, u32, P(dbl), P(u32)]),
         "ng_graph_last_stats": (i32, [vp, P(CallStats)]),
         "ng_cohort_create": (i32, [vp, vp, u32, P(vp)]),
         "ng_cohort_destroy": (None, [vp]),
@@ -308,6 +309,14 @@ class Graph:
         d = ctypes.c_double()
         _check(lib().ng_autotune_delta(self.h, lkind, mode, _ptr(src), len(src), ctypes.byref(d)))
         return d.value
+
+    def autotune_shape(self, sample, lkind: int = NG_LEN_ABS, mode: int = 0):
+        """Tunes bucket width and lanes per workgroup together; returns (delta, block)."""
+        src = np.ascontiguousarray(sample, dtype=np.uint32)
+        d = ctypes.c_double()
+        b = ctypes.c_uint32()
+        _check(lib().ng_autotune(self.h, lkind, mode, _ptr(src), len(src), ctypes.byref(d), ctypes.byref(b)))
+        return d.value, b.value
 
     def last_stats(self) -> dict:
         s = CallStats()

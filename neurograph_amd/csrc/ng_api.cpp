@@ -307,9 +307,20 @@ void free_ws(Workspace& w) {
 // Settle passes expand each bucket's settled nodes in increasing id (a second LDS bitmap) when that
 // costs no resident workgroups: measured +5 % on a 50k-node spatial graph, but a loss wherever the larger
 // LDS footprint lowers occupancy. NG_SORTED_SETTLE=0/1 overrides the choice (tuning experiments).
-bool settle_sorted(DKind dk, int mode, uint32_t n) {
+bool settle_sorted(DKind dk, int mode, uint32_t n, int block) {
     if (const char* e = std::getenv("NG_SORTED_SETTLE")) return std::atoi(e) != 0;
-    return batch_blocks_per_cu(dk, mode, n, true) >= batch_blocks_per_cu(dk, mode, n, false);
+    return batch_blocks_per_cu(dk, mode, n, true, block) >= batch_blocks_per_cu(dk, mode, n, false, block);
+}
+
+// Lanes per workgroup for a call: explicit, else tuned, else NG_BLOCK_SIZE (experiments), else the default.
+int pick_block(const View& v, int mode_idx, int requested) {
+    if (requested == 256 || requested == 512 || requested == 1024) return requested;
+    if (v.tuned_block[mode_idx] > 0) return v.tuned_block[mode_idx];
+    if (const char* e = std::getenv("NG_BLOCK_SIZE")) {
+        const int b = std::atoi(e);
+        if (b == 256 || b == 512 || b == 1024) return b;
+    }
+    return kBlock;
 }
 
 // Pile capacity per list: n first parkings plus 2n re-parkings with improved values.
@@ -338,7 +349,7 @@ uint32_t wanted_pcap(uint32_t n) {
     uint32_t p = pile_capacity(n);
     if (const char* pc = std::getenv("NG_PILE_CAPACITY")) {
         const long long c = std::atoll(pc);
-        if (c > 0) p = static_cast<uint32_t>(std::max<long long>(c, static_cast<long long>(n) + 2 * kBlock));
+        if (c > 0) p = static_cast<uint32_t>(std::max<long long>(c, static_cast<long long>(n) + 2 * kMaxBlock));
     }
     return p;
 }
@@ -434,6 +445,7 @@ struct RunSpec {
     uint32_t n_sources;
     void* dist_out;   // host, optional
     void* dist_dev = nullptr;  // device, optional (n_sources * n elements, kept on the device)
+    int block = 0;             // lanes per workgroup (0: tuned or default)
 };
 
 // Uploads sources, sizes the workspace, launches the batched kernel once, and synchronizes.
@@ -456,8 +468,9 @@ ng_status run_batch(ng_graph* g, const RunSpec& rs, uint32_t* slots_used, double
         v.lend_delta = delta;
     }
     // slots: resident workgroups of this kernel, bounded by the sources and by device memory
-    const bool sorted = settle_sorted(v.dk, rs.mode, g->n);
-    const int bpc = batch_blocks_per_cu(v.dk, rs.mode, g->n, sorted);
+    const int block = pick_block(v, mode_idx, rs.block);
+    const bool sorted = settle_sorted(v.dk, rs.mode, g->n, block);
+    const int bpc = batch_blocks_per_cu(v.dk, rs.mode, g->n, sorted, block);
     uint64_t slots = static_cast<uint64_t>(bpc) * static_cast<uint64_t>(ctx->cus);
     // NG_MAX_SLOTS caps the concurrent source slots (for tuning sweeps and experiments)
     if (const char* cap = std::getenv("NG_MAX_SLOTS")) {
@@ -501,6 +514,7 @@ ng_status run_batch(ng_graph* g, const RunSpec& rs, uint32_t* slots_used, double
     la.dist_out = d_dist_out;
     la.dist_out_kind = dok;
     la.sorted = sorted;
+    la.block = block;
     float ms = 0.0f;
     if (rs.n_sources) {
         NG_HIP(hipEventRecord(ctx->ev0, s));
@@ -533,6 +547,7 @@ ng_status run_batch(ng_graph* g, const RunSpec& rs, uint32_t* slots_used, double
     g->last.kernel_ms = ms;
     g->last.launches = rs.n_sources ? 1 : 0;
     g->last.slots = static_cast<uint32_t>(slots);
+    g->last.block = static_cast<uint32_t>(block);
     g->last.delta = delta;
     g->last.arcs_reached = arcs;
     g->last.flags = err;
@@ -887,6 +902,51 @@ ng_status ng_autotune_delta(ng_graph* g, ng_length_kind lkind, int mode, const u
     return NG_OK;
 }
 
+ng_status ng_autotune(ng_graph* g, ng_length_kind lkind, int mode, const uint32_t* sample, uint32_t n_sample, double* delta_out,
+                      uint32_t* block_out) {
+    if (check_graph(g) != NG_OK) return NG_EINVAL;
+    if (!sample || !n_sample) return fail(NG_EINVAL, "empty sample");
+    const int vi = lkind == NG_LEN_INVERSE ? 1 : 0;
+    ng_status st = ensure_view(g, vi);
+    if (st != NG_OK) return st;
+    View& v = g->views[vi];
+    if (mode != 0 && v.has_zero) return fail(NG_EUNSUPPORTED, "zero-length arcs");
+    const int mi = mode == 0 ? 0 : 1;
+    const int run_mode = mode == 0 ? MODE_SSSP : MODE_BC;
+    const bool integer = v.dk == DKind::U32 || v.dk == DKind::U64;
+    std::vector<double> cand;
+    const double base = v.q05_len > 0.0 ? v.q05_len : (v.min_len > 0.0 ? v.min_len : 1.0);
+    for (double f : {0.25, 0.5, 1.0, 2.0, 4.0, 8.0}) {
+        double d = base * f;
+        if (integer) d = std::max(1.0, std::round(d));
+        if (std::find(cand.begin(), cand.end(), d) == cand.end()) cand.push_back(d);
+    }
+    double best_d = cand.front(), best_ms = std::numeric_limits<double>::infinity();
+    int best_b = kBlock;
+    for (int b : {256, 512, 1024}) {
+        bool first = true;
+        for (double d : cand) {
+            for (int rep = first ? 0 : 1; rep < 2; ++rep) {  // one untimed run per block size (workspace, caches)
+                RunSpec rs{vi, run_mode, d, sample, n_sample, nullptr};
+                rs.block = b;
+                st = run_batch(g, rs, nullptr, nullptr);
+                if (st != NG_OK) return st;
+            }
+            first = false;
+            if (g->last.kernel_ms < best_ms) {
+                best_ms = g->last.kernel_ms;
+                best_d = d;
+                best_b = b;
+            }
+        }
+    }
+    v.tuned_delta[mi] = best_d;
+    v.tuned_block[mi] = best_b;
+    if (delta_out) *delta_out = best_d;
+    if (block_out) *block_out = static_cast<uint32_t>(best_b);
+    return NG_OK;
+}
+
 ng_status ng_graph_last_stats(const ng_graph* g, ng_call_stats* out) {
     if (check_graph(g) != NG_OK) return NG_EINVAL;
     if (!out) return fail(NG_EINVAL, "null output pointer");
@@ -967,8 +1027,8 @@ ng_status cohort_build(ng_cohort* c) {
         gtab[gi].lend = lend;
     }
     // slots
-    const bool sorted = settle_sorted(c->dk, MODE_BC, c->n_max);
-    const int bpc_max = batch_blocks_per_cu(c->dk, MODE_BC, c->n_max, sorted);
+    const bool sorted = settle_sorted(c->dk, MODE_BC, c->n_max, kBlock);
+    const int bpc_max = batch_blocks_per_cu(c->dk, MODE_BC, c->n_max, sorted, kBlock);
     const int bpc = c->bpc > 0 ? std::min(c->bpc, bpc_max) : bpc_max;
     uint64_t slots = static_cast<uint64_t>(bpc) * static_cast<uint64_t>(ctx->cus);
     slots = std::min<uint64_t>(slots, std::max<uint32_t>(c->total, 1));
@@ -1093,6 +1153,7 @@ ng_status cohort_replay(ng_cohort* c) {
     l.kernel_ms = ms;
     l.launches = 1;
     l.slots = c->slots;
+    l.block = kBlock;
     l.delta = 0.0;
     l.arcs_reached = ctrs[0];
     l.flags = err;

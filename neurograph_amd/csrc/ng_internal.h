@@ -11,6 +11,11 @@
 
 namespace ng {
 
+// Lanes per workgroup (one source slot, the frontier tile size): the default and the largest supported; the
+// batched kernel is built for 256, 512 and 1024.
+constexpr int kBlock = 256;
+constexpr int kMaxBlock = 1024;
+
 // Device distance element type: lengths and distances are kept in one of these.
 enum class DKind : int { U32 = 0, U64 = 1, F32 = 2, F64 = 3 };
 
@@ -30,6 +35,7 @@ struct View {
     double min_len = 0, max_len = 0, mean_len = 0, q05_len = 0;  // over finite lengths
     bool has_zero = false;
     double tuned_delta[2] = {0.0, 0.0};  // per mode (0 SSSP, 1 Brandes); 0 = not tuned
+    int tuned_block[2] = {0, 0};         // lanes per workgroup per mode; 0 = not tuned
     double lend_delta = -1.0;
     uint32_t* lend = nullptr;  // [n] end of the light prefix of each row for lend_delta
 };
@@ -126,6 +132,7 @@ struct LaunchArgs {
     void* dist_out;           // device, optional
     int dist_out_kind;        // ng_dist_kind
     bool sorted;              // settle passes expand each bucket's settled nodes in increasing id
+    int block = kBlock;       // lanes per workgroup: 256, 512 or 1024
     // cohort launch (gtab != nullptr; view, sources and n_sources unused; n = the largest member's n)
     const CohortGraph* gtab = nullptr;
     uint32_t n_graphs = 0;
@@ -155,8 +162,8 @@ hipError_t launch_distance_histogram(int dist_kind, const void* dist, const uint
 hipError_t launch_fill_u32(uint32_t* p, uint64_t count, uint32_t value, hipStream_t stream);
 hipError_t launch_fill_f64(double* p, uint64_t count, double value, hipStream_t stream);
 hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t stream);  // bytes % 16 == 0
-// Resident workgroups per CU of the batched kernel for (dk, mode).
-int batch_blocks_per_cu(DKind dk, int mode, uint32_t n, bool sorted);
-constexpr int kBlock = 256;
+// Resident workgroups per CU of the batched kernel for (dk, mode, lanes per workgroup).
+int batch_blocks_per_cu(DKind dk, int mode, uint32_t n, bool sorted, int block);
+
 
 }  // namespace ng

@@ -34,7 +34,6 @@
 namespace ng {
 namespace {
 
-constexpr uint32_t kWaves = kBlock / 64;
 
 // ---------------------------------------------------------------------------------------------------
 // distance types: distances are stored as their bit patterns so that atomic min on the unsigned bits is
@@ -134,7 +133,7 @@ __device__ __forceinline__ T wg_and(T* p, T v) {
 __device__ __forceinline__ void bsync() { __syncthreads(); }
 
 // ---------------------------------------------------------------------------------------------------
-// workgroup primitives (64-wide waves, kBlock lanes)
+// workgroup primitives (64-wide waves, BS lanes per workgroup)
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
@@ -198,19 +197,21 @@ __device__ __forceinline__ void push_entry(bool pred, B val, uint32_t node, Pile
     }
 }
 
-template <class B>
+template <class B, int BS>
 struct Lds {
-    uint32_t t_v[kBlock];
-    uint32_t t_beg[kBlock];
-    uint32_t t_off[kBlock];
-    B t_dv[kBlock];
-    uint32_t wave[kWaves];
+    static constexpr uint32_t kW = BS / 64;
+    uint32_t t_v[BS];
+    uint32_t t_beg[BS];
+    uint32_t t_off[BS];
+    B t_dv[BS];
+    uint32_t wave[kW];
     uint32_t cnt[8];
     B runmin;
-    unsigned long long red_u[3 * kWaves];
-    double red_d[3 * kWaves];
+    unsigned long long red_u[3 * kW];
+    double red_d[3 * kW];
 };
 
+template <int BS>
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t* total, uint32_t* lds_wave) {
     const uint32_t inc = wave_incl_scan(x);
     const uint32_t w = threadIdx.x >> 6;
@@ -218,7 +219,7 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t* total,
     __syncthreads();
     uint32_t base = 0, tot = 0;
 #pragma unroll
-    for (uint32_t i = 0; i < kWaves; ++i) {
+    for (uint32_t i = 0; i < static_cast<uint32_t>(BS / 64); ++i) {
         const uint32_t v = lds_wave[i];
         if (i < w) base += v;
         tot += v;
@@ -231,10 +232,10 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t* total,
 // Expands a list of nodes over per-node arc ranges. node_fn(v, dv&, beg&, cnt&) is called for each listed
 // node (by one lane); arc_fn(v, dv, e) is then called once per arc e in [beg, beg+cnt), with the arcs of
 // a 256-node tile spread over all lanes. Must be called by the whole workgroup with a uniform count.
-template <class B, class NodeFn, class ArcFn>
-__device__ __forceinline__ void expand(const uint32_t* list, uint32_t count, Lds<B>& S, NodeFn node_fn, ArcFn arc_fn) {
+template <int BS, class B, class NodeFn, class ArcFn>
+__device__ __forceinline__ void expand(const uint32_t* list, uint32_t count, Lds<B, BS>& S, NodeFn node_fn, ArcFn arc_fn) {
     const uint32_t tid = threadIdx.x;
-    for (uint32_t base = 0; base < count; base += kBlock) {
+    for (uint32_t base = 0; base < count; base += BS) {
         const uint32_t i = base + tid;
         uint32_t v = 0, beg = 0, cnt = 0;
         B dv = 0;
@@ -243,14 +244,14 @@ __device__ __forceinline__ void expand(const uint32_t* list, uint32_t count, Lds
             node_fn(v, dv, beg, cnt);
         }
         uint32_t total;
-        const uint32_t ex = block_excl_scan(cnt, &total, S.wave);
+        const uint32_t ex = block_excl_scan<BS>(cnt, &total, S.wave);
         S.t_v[tid] = v;
         S.t_dv[tid] = dv;
         S.t_beg[tid] = beg;
         S.t_off[tid] = ex;
         __syncthreads();
-        for (uint32_t k = tid; k < total; k += kBlock) {
-            uint32_t lo = 0, hi = kBlock;
+        for (uint32_t k = tid; k < total; k += BS) {
+            uint32_t lo = 0, hi = BS;
             while (hi - lo > 1) {
                 const uint32_t mid = (lo + hi) >> 1;
                 if (S.t_off[mid] <= k) lo = mid; else hi = mid;
@@ -269,11 +270,11 @@ __device__ __forceinline__ void expand(const uint32_t* list, uint32_t count, Lds
 #ifndef NG_UNROLL
 #define NG_UNROLL 2
 #endif
-template <int U, class B, class L, class NodeFn, class ProbeFn, class ApplyFn>
-__device__ __forceinline__ void expand_relax(const uint32_t* list, uint32_t count, Lds<B>& S, const uint32_t* tgt,
+template <int U, int BS, class B, class L, class NodeFn, class ProbeFn, class ApplyFn>
+__device__ __forceinline__ void expand_relax(const uint32_t* list, uint32_t count, Lds<B, BS>& S, const uint32_t* tgt,
                                              const L* len, NodeFn node_fn, ProbeFn probe, ApplyFn apply) {
     const uint32_t tid = threadIdx.x;
-    for (uint32_t base = 0; base < count; base += kBlock) {
+    for (uint32_t base = 0; base < count; base += BS) {
         const uint32_t i = base + tid;
         uint32_t v = 0, beg = 0, cnt = 0;
         B dv = 0;
@@ -282,21 +283,21 @@ __device__ __forceinline__ void expand_relax(const uint32_t* list, uint32_t coun
             node_fn(v, dv, beg, cnt);
         }
         uint32_t total;
-        const uint32_t ex = block_excl_scan(cnt, &total, S.wave);
+        const uint32_t ex = block_excl_scan<BS>(cnt, &total, S.wave);
         S.t_v[tid] = v;
         S.t_dv[tid] = dv;
         S.t_beg[tid] = beg;
         S.t_off[tid] = ex;
         __syncthreads();
         uint32_t lo_carry = 0;
-        for (uint32_t k0 = tid; k0 < total; k0 += U * kBlock) {
+        for (uint32_t k0 = tid; k0 < total; k0 += U * BS) {
             uint32_t pv[U], pe[U], pt[U];
             B pdv[U], pcur[U];
             L pl[U];
             bool pok[U], pfin[U];
 #pragma unroll
             for (int j = 0; j < U; ++j) {
-                const uint32_t k = k0 + static_cast<uint32_t>(j) * kBlock;
+                const uint32_t k = k0 + static_cast<uint32_t>(j) * BS;
                 pok[j] = k < total;
                 pv[j] = 0;
                 pe[j] = 0;
@@ -306,7 +307,7 @@ __device__ __forceinline__ void expand_relax(const uint32_t* list, uint32_t coun
                 pfin[j] = true;
                 pcur[j] = 0;
                 if (pok[j]) {
-                    uint32_t lo = lo_carry, hi = kBlock;
+                    uint32_t lo = lo_carry, hi = BS;
                     while (hi - lo > 1) {
                         const uint32_t mid = (lo + hi) >> 1;
                         if (S.t_off[mid] <= k) lo = mid; else hi = mid;
@@ -422,18 +423,20 @@ __device__ __forceinline__ uint32_t range_mask(uint32_t wi, uint32_t b, uint32_t
 #define NG_TOCK(var, sec)
 #endif
 
-template <class D, int MODE>
 // Resident waves per SIMD the hot kernel is compiled for (the register budget follows: 512 / waves VGPRs).
 // The path is bound by the latency of random accesses, so occupancy matters more than the few registers
-// spilled to fit.
+// spilled to fit. Workgroups of 512 or 1024 lanes fill a SIMD with 4 waves from one or two workgroups.
 #ifndef NG_WAVES_PER_EU
 #define NG_WAVES_PER_EU 5
 #endif
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NG_WAVES_PER_EU))) void batch_kernel(const KArgs<D> a) {
+template <class D, int MODE, int BS>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? NG_WAVES_PER_EU : 4))) void batch_kernel(
+    const KArgs<D> a) {
     using T = DT<D>;
     using B = typename T::B;
     using L = typename T::L;
-    __shared__ Lds<B> S;
+    __shared__ Lds<B, BS> S;
+    constexpr uint32_t kW = BS / 64;
     // Bitmap (dynamic LDS) of the nodes registered in the settled list of the current source (this bucket or
     // an earlier one). It replaces the global settled stamps, and in a settle pass a heavy arc (length >=
     // delta) into a registered node skips its global distance probe: the target's distance is final and
@@ -500,7 +503,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NG_WAVES
     size_t fpos = cohort ? a.flush_base[slot] : 0;
     auto flush_bc = [&]() {
         if constexpr ((MODE & MODE_BC) != 0) {
-            for (uint32_t v = tid; v < n; v += kBlock) {
+            for (uint32_t v = tid; v < n; v += BS) {
                 a.flush[fpos + v] = R[v].bc;
                 R[v].bc = 0.0;
             }
@@ -534,7 +537,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NG_WAVES
             s = a.sources[si];
         }
         if (phase > 0xE0000000u || gen > 0xE0000000u || pgen > 0xE0000000u) {
-            for (uint32_t v = tid; v < stride; v += kBlock) {
+            for (uint32_t v = tid; v < stride; v += BS) {
                 stc(&fstamp[v], 0u);
                 stc(&sstamp[v], 0u);
             }
@@ -544,9 +547,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NG_WAVES
         ++gen;
         ++phase;
         if (use_bm)
-            for (uint32_t w = tid; w < bmw; w += kBlock) bm[w] = 0u;
+            for (uint32_t w = tid; w < bmw; w += BS) bm[w] = 0u;
         if (sorted)
-            for (uint32_t w = tid; w < bmw; w += kBlock) bn[w] = 0u;
+            for (uint32_t w = tid; w < bmw; w += BS) bn[w] = 0u;
         if (tid == 0) {
 #pragma unroll
             for (int k = 0; k < 8; ++k) S.cnt[k] = 0;
@@ -604,7 +607,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NG_WAVES
                         // reloads the distances (exact mode)
                         if (old == T::kInf) {
                             pp = true;
-                        } else if (S.cnt[1] + kBlock < pcap - n) {
+                        } else if (S.cnt[1] + BS < pcap - n) {
                             pp = true;
                         } else {
                             S.cnt[5] = 1u;
@@ -640,7 +643,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NG_WAVES
                 const uint32_t nph = phase + 1;
                 ++c_phases;
                 NG_TICK(t_inner);
-                expand_relax<NG_UNROLL>(
+                expand_relax<NG_UNROLL, BS>(
                     Fc, nF, S, tgt, len,
                     [&](uint32_t v, B& dv, uint32_t& beg, uint32_t& cnt) {
                         dv = ldc(DP(v));
@@ -653,7 +656,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NG_WAVES
                         apply(ok, v, dv, e, t, l, fin, cur, nph, false);
                     });
                 // first visit in this bucket -> settled list
-                for (uint32_t base = 0; base < nF; base += kBlock) {
+                for (uint32_t base = 0; base < nF; base += BS) {
                     const uint32_t i = base + tid;
                     bool fresh = false;
                     uint32_t v = 0;
@@ -687,11 +690,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NG_WAVES
             if (aborted) break;
             if (sorted && nS > settle_from) {
                 uint32_t pos = settle_from;
-                for (uint32_t wb = 0; wb < bmw; wb += kBlock) {
+                for (uint32_t wb = 0; wb < bmw; wb += BS) {
                     const uint32_t wi = wb + tid;
                     uint32_t w = wi < bmw ? bn[wi] : 0u;
                     uint32_t tot;
-                    const uint32_t ex = block_excl_scan(static_cast<uint32_t>(__popc(w)), &tot, S.wave);
+                    const uint32_t ex = block_excl_scan<BS>(static_cast<uint32_t>(__popc(w)), &tot, S.wave);
                     if (w != 0u) {
                         bn[wi] = 0u;
                         uint32_t k = pos + ex;
@@ -710,7 +713,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NG_WAVES
             if (nS > settle_from) {
                 NG_TICK(t_settle);
                 const uint32_t nph = phase + 1;
-                expand_relax<NG_UNROLL>(
+                expand_relax<NG_UNROLL, BS>(
                     Sl + settle_from, nS - settle_from, S, tgt, len,
                     [&](uint32_t v, B& dv, uint32_t& beg, uint32_t& cnt) {
                         dv = ldc(DP(v));
@@ -768,7 +771,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NG_WAVES
             bsync();
             // split the pile: entries of settled nodes are dropped, values below hi join the frontier
             // (duplicates removed by the frontier stamp), the rest is carried over with its value
-            for (uint32_t base = 0; base < nP; base += kBlock) {
+            for (uint32_t base = 0; base < nP; base += BS) {
                 const uint32_t i = base + tid;
                 bool tf = false, tp = false;
                 uint32_t t = 0;
@@ -814,7 +817,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NG_WAVES
         {
             unsigned long long su = 0, mx = 0, arcs = 0;
             double sd = 0.0, se = 0.0;
-            for (uint32_t i = tid; i < nS; i += kBlock) {
+            for (uint32_t i = tid; i < nS; i += BS) {
                 const uint32_t v = Sl[i];
                 const B d = ldc(DP(v));
                 if constexpr (T::kInt) su += static_cast<unsigned long long>(d);
@@ -834,21 +837,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NG_WAVES
             const uint32_t w = tid >> 6;
             if (lane_id() == 0) {
                 S.red_u[w] = su;
-                S.red_u[kWaves + w] = mx;
-                S.red_u[2 * kWaves + w] = arcs;
+                S.red_u[kW + w] = mx;
+                S.red_u[2 * kW + w] = arcs;
                 S.red_d[w] = sd;
-                S.red_d[kWaves + w] = se;
+                S.red_d[kW + w] = se;
             }
             __syncthreads();
             if (tid == 0) {
                 unsigned long long tsu = 0, tmx = 0, tar = 0;
                 double tsd = 0.0, tse = 0.0;
-                for (uint32_t k = 0; k < kWaves; ++k) {
+                for (uint32_t k = 0; k < kW; ++k) {
                     tsu += S.red_u[k];
-                    tmx = S.red_u[kWaves + k] > tmx ? S.red_u[kWaves + k] : tmx;
-                    tar += S.red_u[2 * kWaves + k];
+                    tmx = S.red_u[kW + k] > tmx ? S.red_u[kW + k] : tmx;
+                    tar += S.red_u[2 * kW + k];
                     tsd += S.red_d[k];
-                    tse += S.red_d[kWaves + k];
+                    tse += S.red_d[kW + k];
                 }
                 ng_sssp_summary r;
                 if constexpr (T::kInt) {
@@ -883,13 +886,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NG_WAVES
                 }
             };
             if (nlog <= a.logcap) {
-                for (uint32_t i = tid; i < nlog; i += kBlock) {
+                for (uint32_t i = tid; i < nlog; i += BS) {
                     const LogEntry r = log[i];
                     mark(r.t, r.e, static_cast<B>(r.nd));
                 }
             } else {
                 if (tid == 0) atomicOr(a.err, ERR_LOG_OVERFLOW);
-                expand(
+                expand<BS>(
                     Sl, nS, S,
                     [&](uint32_t v, B& dv, uint32_t& beg, uint32_t& cnt) {
                         dv = 0;
@@ -912,7 +915,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NG_WAVES
             while (lb < le && nl < n) {
                 if (tid == 0) layers[nl] = lb;
                 ++nl;
-                for (uint32_t i = lb + tid; i < le; i += kBlock) {
+                for (uint32_t i = lb + tid; i < le; i += BS) {
                     const uint32_t u = order[i];
                     const double su = ldc(&R[u].sigma);
                     const uint32_t b = off[u], e_end = off[u + 1];
@@ -941,7 +944,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NG_WAVES
             c_layers += nl;
             if (le != nS) {  // a zero-length tight cycle kept some nodes from being released
                 if (tid == 0) atomicOr(a.err, ERR_KAHN);
-                for (uint32_t i = tid; i < nS; i += kBlock) {
+                for (uint32_t i = tid; i < nS; i += BS) {
                     const uint32_t u = Sl[i];
                     stc(&R[u].indeg, 0u);
                     const uint32_t b = off[u], e_end = off[u + 1];
@@ -955,7 +958,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NG_WAVES
             // ---- reverse layers: dependencies, per-node and per-arc accumulation, bitmap cleared
             for (int ly = static_cast<int>(nl) - 1; ly >= 0; --ly) {
                 const uint32_t l0 = layers[ly], l1 = layers[ly + 1];
-                for (uint32_t i = l0 + tid; i < l1; i += kBlock) {
+                for (uint32_t i = l0 + tid; i < l1; i += BS) {
                     const uint32_t u = order[i];
                     const double su = ldc(&R[u].sigma);
                     double acc1 = 0.0, acc2 = 0.0;
@@ -995,16 +998,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NG_WAVES
 
         NG_TICK(t_reset);
         if (a.dist_out != nullptr) {
-            for (uint32_t v = tid; v < n; v += kBlock)
+            for (uint32_t v = tid; v < n; v += BS)
                 write_dist_out(a, static_cast<size_t>(si) * n + v, ldc(DP(v)));
             bsync();  // every lane has read its distances before any lane resets them below
         }
         // ---- restore the slot: only reached nodes were touched (after an abort, frontier and pile nodes
         // may hold distances too, so the whole array is reset)
         if (aborted) {
-            for (uint32_t v = tid; v < n; v += kBlock) stc(DP(v), T::kInf);
+            for (uint32_t v = tid; v < n; v += BS) stc(DP(v), T::kInf);
         }
-        for (uint32_t i = tid; i < nS; i += kBlock) {
+        for (uint32_t i = tid; i < nS; i += BS) {
             const uint32_t v = Sl[i];
             stc(DP(v), T::kInf);
             if constexpr (MODE != MODE_SSSP) {
@@ -1094,7 +1097,7 @@ uint32_t bitmap_words(uint32_t n, bool sorted) {
     return w * 4 * (sorted ? 2 : 1) <= 96 * 1024 ? w : 0u;
 }
 
-template <class D, int MODE>
+template <class D, int MODE, int BS>
 hipError_t launch_typed(const LaunchArgs& la, hipStream_t stream) {
     using T = DT<D>;
     using B = typename T::B;
@@ -1149,56 +1152,75 @@ hipError_t launch_typed(const LaunchArgs& la, hipStream_t stream) {
     k.dist_out_kind = la.dist_out_kind;
     k.bm_words = bitmap_words(la.n, la.sorted);
     k.sorted = (la.sorted && k.bm_words) ? 1u : 0u;
-    hipLaunchKernelGGL((batch_kernel<D, MODE>), dim3(la.grid), dim3(kBlock), k.bm_words * 4 * (k.sorted ? 2 : 1), stream, k);
+    hipLaunchKernelGGL((batch_kernel<D, MODE, BS>), dim3(la.grid), dim3(BS), k.bm_words * 4 * (k.sorted ? 2 : 1), stream, k);
     return hipGetLastError();
 }
 
-template <class D>
+template <class D, int BS>
 hipError_t launch_mode(const LaunchArgs& a, hipStream_t stream) {
     switch (a.mode) {
-    case MODE_SSSP: return launch_typed<D, MODE_SSSP>(a, stream);
-    case MODE_BC: return launch_typed<D, MODE_BC>(a, stream);
-    case MODE_ARC1: return launch_typed<D, MODE_ARC1>(a, stream);
-    case MODE_ARC1 | MODE_ARC2: return launch_typed<D, MODE_ARC1 | MODE_ARC2>(a, stream);
+    case MODE_SSSP: return launch_typed<D, MODE_SSSP, BS>(a, stream);
+    case MODE_BC: return launch_typed<D, MODE_BC, BS>(a, stream);
+    case MODE_ARC1: return launch_typed<D, MODE_ARC1, BS>(a, stream);
+    case MODE_ARC1 | MODE_ARC2: return launch_typed<D, MODE_ARC1 | MODE_ARC2, BS>(a, stream);
     default: return hipErrorInvalidValue;
     }
 }
 
 template <class D>
-int blocks_per_cu_mode(int mode, uint32_t n, bool sorted) {
+hipError_t launch_block(const LaunchArgs& a, hipStream_t stream) {
+    switch (a.block) {
+    case 256: return launch_mode<D, 256>(a, stream);
+    case 512: return launch_mode<D, 512>(a, stream);
+    case 1024: return launch_mode<D, 1024>(a, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <class D, int BS>
+int blocks_per_cu_bs(int mode, size_t dyn) {
     int nb = 0;
     hipError_t e = hipErrorInvalidValue;
-    const uint32_t bw = bitmap_words(n, sorted);
-    const size_t dyn = static_cast<size_t>(bw) * 4 * ((sorted && bw) ? 2 : 1);
     switch (mode) {
-    case MODE_SSSP: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<D, MODE_SSSP>, kBlock, dyn); break;
-    case MODE_BC: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<D, MODE_BC>, kBlock, dyn); break;
-    case MODE_ARC1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<D, MODE_ARC1>, kBlock, dyn); break;
+    case MODE_SSSP: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<D, MODE_SSSP, BS>, BS, dyn); break;
+    case MODE_BC: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<D, MODE_BC, BS>, BS, dyn); break;
+    case MODE_ARC1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<D, MODE_ARC1, BS>, BS, dyn); break;
     default:
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<D, MODE_ARC1 | MODE_ARC2>, kBlock, dyn);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<D, MODE_ARC1 | MODE_ARC2, BS>, BS, dyn);
         break;
     }
     return (e == hipSuccess && nb > 0) ? nb : 1;
+}
+
+template <class D>
+int blocks_per_cu_mode(int mode, uint32_t n, bool sorted, int block) {
+    const uint32_t bw = bitmap_words(n, sorted);
+    const size_t dyn = static_cast<size_t>(bw) * 4 * ((sorted && bw) ? 2 : 1);
+    switch (block) {
+    case 512: return blocks_per_cu_bs<D, 512>(mode, dyn);
+    case 1024: return blocks_per_cu_bs<D, 1024>(mode, dyn);
+    default: return blocks_per_cu_bs<D, 256>(mode, dyn);
+    }
 }
 
 }  // namespace
 
 hipError_t launch_batch(const LaunchArgs& a, hipStream_t stream) {
     switch (a.dk) {
-    case DKind::U32: return launch_mode<uint32_t>(a, stream);
-    case DKind::U64: return launch_mode<uint64_t>(a, stream);
-    case DKind::F32: return launch_mode<float>(a, stream);
-    case DKind::F64: return launch_mode<double>(a, stream);
+    case DKind::U32: return launch_block<uint32_t>(a, stream);
+    case DKind::U64: return launch_block<uint64_t>(a, stream);
+    case DKind::F32: return launch_block<float>(a, stream);
+    case DKind::F64: return launch_block<double>(a, stream);
     }
     return hipErrorInvalidValue;
 }
 
-int batch_blocks_per_cu(DKind dk, int mode, uint32_t n, bool sorted) {
+int batch_blocks_per_cu(DKind dk, int mode, uint32_t n, bool sorted, int block) {
     switch (dk) {
-    case DKind::U32: return blocks_per_cu_mode<uint32_t>(mode, n, sorted);
-    case DKind::U64: return blocks_per_cu_mode<uint64_t>(mode, n, sorted);
-    case DKind::F32: return blocks_per_cu_mode<float>(mode, n, sorted);
-    case DKind::F64: return blocks_per_cu_mode<double>(mode, n, sorted);
+    case DKind::U32: return blocks_per_cu_mode<uint32_t>(mode, n, sorted, block);
+    case DKind::U64: return blocks_per_cu_mode<uint64_t>(mode, n, sorted, block);
+    case DKind::F32: return blocks_per_cu_mode<float>(mode, n, sorted, block);
+    case DKind::F64: return blocks_per_cu_mode<double>(mode, n, sorted, block);
     }
     return 1;
 }

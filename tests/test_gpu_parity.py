@@ -308,3 +308,36 @@ def test_distance_histogram_and_matrix(ctx, wdtype):
     b = np.minimum(np.floor(d / width), bins - 1).astype(np.int64)
     np.testing.assert_array_equal(counts, np.bincount(b, minlength=bins).astype(np.uint64))
     assert int(counts.sum()) == len(srcs) * (hg.n - 1) - int(summ["n_unreached"].sum())
+
+
+@pytest.mark.parametrize("block", ["256", "512", "1024"])
+def test_workgroup_sizes(ctx, monkeypatch, block):
+    """Every workgroup size the kernel is built for (one source per workgroup of 256, 512 or 1024 lanes) gives
+    the oracle's distances bit-exactly and its betweenness and edge path counts within 1e-9."""
+    import neurograph_amd as ng
+
+    monkeypatch.setenv("NG_BLOCK_SIZE", block)
+    for wdtype, seed in ((np.int32, 15), (np.float32, 16), (np.float64, 17)):
+        c = rand_graph(3000, 30000, seed, wdtype=wdtype, wlo=1, whi=9)
+        g = dev_graph(ctx, c)
+        srcs = np.arange(0, 3000, 23)
+        ref = np.stack([orc.sssp(c, int(s))[0] for s in srcs])
+        _, dist = g.sssp(srcs, delta=0.8, want_dist=True)
+        np.testing.assert_array_equal(dist, ref)
+        assert g.last_stats()["block"] == int(block)
+        bc = g.brandes(srcs)
+        np.testing.assert_allclose(bc, orc.brandes(c, sources=srcs)["bc"], rtol=1e-9, atol=1e-9)
+        edge = g.edge_dependency(srcs)
+        np.testing.assert_allclose(edge, orc.brandes(c, sources=srcs, edge=True)["edge"], rtol=1e-9, atol=1e-12)
+
+
+def test_autotune_shape(ctx):
+    import neurograph_amd as ng
+
+    c = rand_graph(5000, 60000, 21, wdtype=np.float32)
+    g = dev_graph(ctx, c)
+    d, b = g.autotune_shape(np.arange(0, 5000, 2), mode=1)
+    assert d > 0 and b in (256, 512, 1024)
+    srcs = np.arange(0, 5000, 41)
+    g.brandes(srcs)
+    assert g.last_stats()["block"] == b and g.last_stats()["delta"] == d
