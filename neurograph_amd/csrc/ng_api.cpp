@@ -304,12 +304,16 @@ void free_ws(Workspace& w) {
     w.generation = gen + 1;
 }
 
-// Settle passes expand each bucket's settled nodes in increasing id (a second LDS bitmap) when that
-// costs no resident workgroups: measured +5 % on a 50k-node spatial graph, but a loss wherever the larger
-// LDS footprint lowers occupancy. NG_SORTED_SETTLE=0/1 overrides the choice (tuning experiments).
-bool settle_sorted(DKind dk, int mode, uint32_t n, int block) {
-    if (const char* e = std::getenv("NG_SORTED_SETTLE")) return std::atoi(e) != 0;
-    return batch_blocks_per_cu(dk, mode, n, true, block) >= batch_blocks_per_cu(dk, mode, n, false, block);
+// Extra LDS bitmaps, each used only when it costs no resident workgroups: MAP_SORTED makes settle passes
+// expand each bucket's settled nodes in increasing id (measured +5 % on a 50k-node spatial graph, a loss
+// wherever its LDS lowers occupancy); MAP_FRONTIER replaces the global frontier stamps. NG_LDS_MAPS=0..3
+// overrides the choice (tuning experiments).
+int lds_maps_for(DKind dk, int mode, uint32_t n, int block) {
+    if (const char* e = std::getenv("NG_LDS_MAPS")) return std::atoi(e) & 3;
+    const int base = batch_blocks_per_cu(dk, mode, n, 0, block);
+    for (int maps : {static_cast<int>(MAP_SORTED | MAP_FRONTIER), static_cast<int>(MAP_FRONTIER), static_cast<int>(MAP_SORTED)})
+        if (batch_blocks_per_cu(dk, mode, n, maps, block) >= base) return maps;
+    return 0;
 }
 
 // Lanes per workgroup for a call: explicit, else tuned, else NG_BLOCK_SIZE (experiments), else the default.
@@ -469,8 +473,8 @@ ng_status run_batch(ng_graph* g, const RunSpec& rs, uint32_t* slots_used, double
     }
     // slots: resident workgroups of this kernel, bounded by the sources and by device memory
     const int block = pick_block(v, mode_idx, rs.block);
-    const bool sorted = settle_sorted(v.dk, rs.mode, g->n, block);
-    const int bpc = batch_blocks_per_cu(v.dk, rs.mode, g->n, sorted, block);
+    const int maps = lds_maps_for(v.dk, rs.mode, g->n, block);
+    const int bpc = batch_blocks_per_cu(v.dk, rs.mode, g->n, maps, block);
     uint64_t slots = static_cast<uint64_t>(bpc) * static_cast<uint64_t>(ctx->cus);
     // NG_MAX_SLOTS caps the concurrent source slots (for tuning sweeps and experiments)
     if (const char* cap = std::getenv("NG_MAX_SLOTS")) {
@@ -513,7 +517,7 @@ ng_status run_batch(ng_graph* g, const RunSpec& rs, uint32_t* slots_used, double
     la.ws = &w;
     la.dist_out = d_dist_out;
     la.dist_out_kind = dok;
-    la.sorted = sorted;
+    la.maps = maps;
     la.block = block;
     float ms = 0.0f;
     if (rs.n_sources) {
@@ -1027,8 +1031,8 @@ ng_status cohort_build(ng_cohort* c) {
         gtab[gi].lend = lend;
     }
     // slots
-    const bool sorted = settle_sorted(c->dk, MODE_BC, c->n_max, kBlock);
-    const int bpc_max = batch_blocks_per_cu(c->dk, MODE_BC, c->n_max, sorted, kBlock);
+    const int maps = lds_maps_for(c->dk, MODE_BC, c->n_max, kBlock);
+    const int bpc_max = batch_blocks_per_cu(c->dk, MODE_BC, c->n_max, maps, kBlock);
     const int bpc = c->bpc > 0 ? std::min(c->bpc, bpc_max) : bpc_max;
     uint64_t slots = static_cast<uint64_t>(bpc) * static_cast<uint64_t>(ctx->cus);
     slots = std::min<uint64_t>(slots, std::max<uint32_t>(c->total, 1));
@@ -1094,7 +1098,7 @@ ng_status cohort_build(ng_cohort* c) {
     la.ws = &w;
     la.dist_out = nullptr;
     la.dist_out_kind = 0;
-    la.sorted = sorted;
+    la.maps = maps;
     la.gtab = c->d_gtab;
     la.n_graphs = G;
     la.node_prefix = c->d_prefix;

@@ -372,7 +372,7 @@ struct KArgs {
     double* flush;
     const uint64_t* flush_base;
     uint32_t bm_words;  // words of the LDS bitmap of finalized nodes (0: no bitmap)
-    uint32_t sorted;    // 1: a second LDS bitmap orders each bucket's settled nodes by id
+    uint32_t maps;      // extra LDS bitmaps: bit 0 orders each bucket's settled nodes by id, bit 1 marks the next frontier
     double* grad1;
     double* grad2;
     ng_sssp_summary* summaries;
@@ -447,8 +447,17 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
     // the bucket's part of the settled list is written by enumerating bn, i.e. in increasing node id, so
     // that consecutive tiles of the settle pass expand nodes with nearby ids (and, for a locality-ordered
     // numbering, nearby targets).
-    const bool sorted = use_bm && a.sorted != 0;
+    const bool sorted = use_bm && (a.maps & 1u) != 0;
     uint32_t* const bn = bm + a.bm_words;
+    // With `use_fb`, a third bitmap marks the nodes already pushed to the next frontier (replacing the global
+    // frontier stamps): a frontier's bits are cleared when its phase starts, so a node improved again in that
+    // phase can be pushed once more.
+    const bool use_fb = use_bm && (a.maps & 2u) != 0;
+    uint32_t* const fb = bm + a.bm_words * (sorted ? 2u : 1u);
+    auto fb_first = [fb](uint32_t t) -> bool {
+        const uint32_t bit = 1u << (t & 31);
+        return (atomicOr(&fb[t >> 5], bit) & bit) == 0u;
+    };
 
     const uint32_t tid = threadIdx.x;
     const uint32_t slot = blockIdx.x;
@@ -550,6 +559,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
             for (uint32_t w = tid; w < bmw; w += BS) bm[w] = 0u;
         if (sorted)
             for (uint32_t w = tid; w < bmw; w += BS) bn[w] = 0u;
+        if (use_fb)
+            for (uint32_t w = tid; w < bmw; w += BS) fb[w] = 0u;
         if (tid == 0) {
 #pragma unroll
             for (int k = 0; k < 8; ++k) S.cnt[k] = 0;
@@ -598,7 +609,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
                 if (nd < old) {
                     ++c_improve;
                     if (nd < hi) {
-                        pf = wg_exch(&fstamp[t], nph) != nph;
+                        pf = use_fb ? fb_first(t) : wg_exch(&fstamp[t], nph) != nph;
                     } else {
                         atomicMin(&S.runmin, nd);
                         // first parking of t in this source (its first reach: every later reach finds a finite
@@ -643,6 +654,13 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
                 const uint32_t nph = phase + 1;
                 ++c_phases;
                 NG_TICK(t_inner);
+                if (use_fb) {
+                    for (uint32_t i = tid; i < nF; i += BS) {
+                        const uint32_t v = Fc[i];
+                        atomicAnd(&fb[v >> 5], ~(1u << (v & 31)));
+                    }
+                    bsync();
+                }
                 expand_relax<NG_UNROLL, BS>(
                     Fc, nF, S, tgt, len,
                     [&](uint32_t v, B& dv, uint32_t& beg, uint32_t& cnt) {
@@ -783,7 +801,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
                     if (!settled) {
                         d = exact ? ldc(DP(t)) : pe.val;
                         if (d < hi) {
-                            tf = wg_exch(&fstamp[t], nph) != nph;
+                            tf = use_fb ? fb_first(t) : wg_exch(&fstamp[t], nph) != nph;
                         } else {
                             atomicMin(&S.runmin, d);
                             tp = true;
@@ -1090,11 +1108,13 @@ hipError_t fill(T* p, uint64_t count, T value, hipStream_t stream) {
     return hipGetLastError();
 }
 
-// Words of the finalized-node bitmap kept in LDS, or 0 when it (and, with `sorted`, its twin) would not fit
-// next to the static LDS.
-uint32_t bitmap_words(uint32_t n, bool sorted) {
+// LDS bitmaps of the batched kernel: the finalized-node bitmap plus one per bit of `maps`.
+uint32_t lds_maps(int maps) { return 1u + (maps & 1 ? 1u : 0u) + (maps & 2 ? 1u : 0u); }
+
+// Words of each LDS bitmap, or 0 when the bitmaps would not fit next to the static LDS.
+uint32_t bitmap_words(uint32_t n, int maps) {
     const uint32_t w = (n + 31) / 32;
-    return w * 4 * (sorted ? 2 : 1) <= 96 * 1024 ? w : 0u;
+    return static_cast<uint64_t>(w) * 4 * lds_maps(maps) <= 96 * 1024 ? w : 0u;
 }
 
 template <class D, int MODE, int BS>
@@ -1150,9 +1170,10 @@ hipError_t launch_typed(const LaunchArgs& la, hipStream_t stream) {
     k.err = ws.err;
     k.dist_out = la.dist_out;
     k.dist_out_kind = la.dist_out_kind;
-    k.bm_words = bitmap_words(la.n, la.sorted);
-    k.sorted = (la.sorted && k.bm_words) ? 1u : 0u;
-    hipLaunchKernelGGL((batch_kernel<D, MODE, BS>), dim3(la.grid), dim3(BS), k.bm_words * 4 * (k.sorted ? 2 : 1), stream, k);
+    k.bm_words = bitmap_words(la.n, la.maps);
+    k.maps = k.bm_words ? static_cast<uint32_t>(la.maps) : 0u;
+    hipLaunchKernelGGL((batch_kernel<D, MODE, BS>), dim3(la.grid), dim3(BS), k.bm_words * 4 * lds_maps(static_cast<int>(k.maps)),
+                       stream, k);
     return hipGetLastError();
 }
 
@@ -1193,9 +1214,9 @@ int blocks_per_cu_bs(int mode, size_t dyn) {
 }
 
 template <class D>
-int blocks_per_cu_mode(int mode, uint32_t n, bool sorted, int block) {
-    const uint32_t bw = bitmap_words(n, sorted);
-    const size_t dyn = static_cast<size_t>(bw) * 4 * ((sorted && bw) ? 2 : 1);
+int blocks_per_cu_mode(int mode, uint32_t n, int maps, int block) {
+    const uint32_t bw = bitmap_words(n, maps);
+    const size_t dyn = static_cast<size_t>(bw) * 4 * (bw ? lds_maps(maps) : 0u);
     switch (block) {
     case 512: return blocks_per_cu_bs<D, 512>(mode, dyn);
     case 1024: return blocks_per_cu_bs<D, 1024>(mode, dyn);
@@ -1215,12 +1236,12 @@ hipError_t launch_batch(const LaunchArgs& a, hipStream_t stream) {
     return hipErrorInvalidValue;
 }
 
-int batch_blocks_per_cu(DKind dk, int mode, uint32_t n, bool sorted, int block) {
+int batch_blocks_per_cu(DKind dk, int mode, uint32_t n, int maps, int block) {
     switch (dk) {
-    case DKind::U32: return blocks_per_cu_mode<uint32_t>(mode, n, sorted, block);
-    case DKind::U64: return blocks_per_cu_mode<uint64_t>(mode, n, sorted, block);
-    case DKind::F32: return blocks_per_cu_mode<float>(mode, n, sorted, block);
-    case DKind::F64: return blocks_per_cu_mode<double>(mode, n, sorted, block);
+    case DKind::U32: return blocks_per_cu_mode<uint32_t>(mode, n, maps, block);
+    case DKind::U64: return blocks_per_cu_mode<uint64_t>(mode, n, maps, block);
+    case DKind::F32: return blocks_per_cu_mode<float>(mode, n, maps, block);
+    case DKind::F64: return blocks_per_cu_mode<double>(mode, n, maps, block);
     }
     return 1;
 }

@@ -249,11 +249,12 @@ def test_small_pile_forces_exact_bucket_advance(ctx, monkeypatch):
         np.testing.assert_allclose(bc, orc.brandes(c, sources=srcs)["bc"], rtol=1e-9, atol=1e-9)
 
 
-@pytest.mark.parametrize("order", ["0", "1"])
-def test_settle_order_variants(ctx, monkeypatch, order):
-    """Both settle-pass orders (discovery order, and increasing id through the second LDS bitmap) give the
-    oracle's distances bit-exactly and its betweenness within 1e-9."""
-    monkeypatch.setenv("NG_SORTED_SETTLE", order)
+@pytest.mark.parametrize("maps", ["0", "1", "2", "3"])
+def test_lds_bitmap_variants(ctx, monkeypatch, maps):
+    """Every combination of the optional LDS bitmaps (settle passes in discovery order or in increasing id;
+    frontier de-duplication by global stamps or by an LDS bitmap) gives the oracle's distances bit-exactly and
+    its betweenness within 1e-9."""
+    monkeypatch.setenv("NG_LDS_MAPS", maps)
     for wdtype, seed in ((np.int32, 5), (np.float64, 6)):
         c = rand_graph(4000, 40000, seed, wdtype=wdtype, wlo=1, whi=9)
         g = dev_graph(ctx, c)
