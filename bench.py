@@ -67,6 +67,27 @@ def bytes_per_source(g) -> float:
     return 2.0 * (g.m * (4 + s_w) + (g.n + 1) * 4) + g.n * (2 * s_d + 32)
 
 
+def read_traffic(workload: str):
+    """HBM bytes per launch of the hot kernel from the committed rocprofv3 counter summary of this workload
+    (FETCH_SIZE + WRITE_SIZE, each scaled by what the counter reads for a copy of known size), or None."""
+    prof = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(prof):
+        return None
+    with open(prof) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def copy_calibration(ctx, mib: int):
+    """One device copy of a known size (outside the timed region): the reference kernel for scaling profiler
+    byte counters."""
+    if mib <= 0:
+        return
+    nb = mib << 20
+    cms = ctx.copy_calibration(nb)
+    print(f"[bench] copy_kernel {nb} bytes read + {nb} written in {cms:.3f} ms ({2 * nb / cms / 1e6:.1f} GB/s)",
+          file=sys.stderr, flush=True)
+
+
 def make_cohort(seed: int = 0):
     """64 member graphs: n ~ U[1000, 20000], mean out-degree ~ U[10, 60] (seeded), spatial generator, FP32."""
     import neurograph_amd as ng
@@ -106,6 +127,7 @@ def run_cohort(args):
         arcs += co.last_stats()["arcs_reached"]
     ctx.synchronize()
     wall = time.perf_counter() - t0
+    copy_calibration(ctx, args.copy_calibration_mib)
     # the same work as 64 plain launches (each graph's own tuned width, all its sources, one call per graph)
     for g in gs:
         g.brandes(delta=0.0)
@@ -122,17 +144,14 @@ def run_cohort(args):
     plain_value = n_total * steps / plain_wall
     algo = sum(bytes_per_source(h) * h.n for h in hgs)
     achieved = algo * steps / (sum(kernel_ms) / 1e3) / 1e9
-    traffic = None
-    prof = os.path.join(ROOT, "profiles", "traffic_cohort64.json")
-    if os.path.exists(prof):
-        with open(prof) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+    traffic = read_traffic("cohort64")
     res = {
         "metric": METRIC, "value": round(value, 3), "unit": "sources/s", "n_gpus": 1, "steps": steps,
         "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / steps, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "cohort64", "graphs": 64, "nodes": "U[1000,20000]", "mean_out_degree": "U[10,60]",
                    "weights": "f32 lognormal(0,0.5)", "sources": n_total, "arcs": m_total,
+                   "lanes_per_source": co.last_stats()["block"], "slots": co.last_stats()["slots"],
                    "outputs": "per-source APSP summaries + betweenness of every graph in one replay"},
         "gteps": round(arcs / wall / 1e9, 3),
         "kernel_ms_per_step": round(sum(kernel_ms) / steps, 3),
@@ -157,9 +176,9 @@ def run_adjoint(args):
     ctx = ng.Context(0)
     g = ng.Graph(ctx, hg)
     src = np.arange(hg.n, dtype=np.uint32)
-    sample = np.arange(0, hg.n, 40, dtype=np.uint32)
-    g.autotune(sample, mode=1)
-    g.autotune(sample, mode=0)
+    sample = np.arange(0, hg.n, 6, dtype=np.uint32)
+    g.autotune_shape(sample, mode=1)
+    g.autotune_shape(sample, mode=0)
     steps = args.steps or 3
     for _ in range(args.warmup):
         g.adjoint(src)
@@ -177,6 +196,7 @@ def run_adjoint(args):
         ctx.synchronize()
         times[name] = (time.perf_counter() - t0) / steps
         kms[name] = k / steps
+    copy_calibration(ctx, args.copy_calibration_mib)
     value = hg.n / times["adjoint"]
     # adjoint traffic model: B_bc plus the gradient read-modify-write of the two per-arc vectors
     algo = bytes_per_source(hg) * hg.n + 2 * 16.0 * hg.m
@@ -186,13 +206,14 @@ def run_adjoint(args):
         "ms_per_step": round(times["adjoint"] * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": "adjoint20k", "nodes": hg.n, "arcs": hg.m, "mean_out_degree": 30, "weights": "f64 U[0.5,1.5]",
+                   "lanes_per_source": g.last_stats()["block"],
                    "outputs": "dCPL/dw and dGE/dw for every arc, all sources"},
         "kernel_ms_per_step": round(kms["adjoint"], 3),
         "plain_forward": {"value": round(hg.n / times["forward"], 3), "unit": "sources/s", "kernel_ms": round(kms["forward"], 3)},
         "plain_brandes": {"value": round(hg.n / times["brandes"], 3), "unit": "sources/s", "kernel_ms": round(kms["brandes"], 3)},
         "adjoint_vs_forward": round(times["forward"] / times["adjoint"], 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic("adjoint20k")},
     }
     print(json.dumps(res), flush=True)
 
@@ -323,19 +344,10 @@ def main():
     wall = time.perf_counter() - t0
 
     value = srcs / wall
-    if args.copy_calibration_mib > 0:
-        nb = args.copy_calibration_mib << 20
-        cms = ctx.copy_calibration(nb)
-        print(f"[bench] copy_kernel {nb} bytes read + {nb} written in {cms:.3f} ms "
-              f"({2 * nb / cms / 1e6:.1f} GB/s)", file=sys.stderr, flush=True)
+    copy_calibration(ctx, args.copy_calibration_mib)
     launches_bytes = [bytes_per_source(hg) * len(batches[k % len(batches)]) for k in range(steps)]
     achieved = sum(launches_bytes) / (sum(kernel_ms) / 1e3) / 1e9
-    traffic = None
-    prof = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
-    if os.path.exists(prof):
-        with open(prof) as f:
-            tr = json.load(f)
-        traffic = tr.get("hbm_bytes_per_launch")
+    traffic = read_traffic(args.workload)
     res = {
         "metric": METRIC,
         "value": round(value, 3),

@@ -182,8 +182,9 @@ void ng_cohort_destroy(ng_cohort* c);
  * graph order; kernel_ms (optional): duration of the whole replay (HIP events). */
 ng_status ng_cohort_brandes(ng_cohort* c, double* bc_out, ng_sssp_summary* summaries_out, double* kernel_ms);
 /* Autotuner of the cohort launch: per member graph the bucket width (ng_autotune_delta in Brandes mode on
- * n_sample evenly spaced sources), then the resident workgroups per CU (occupancy maximum and two lower
- * values, one timed replay each). The next ng_cohort_brandes recaptures with the chosen shapes. */
+ * n_sample evenly spaced sources), then the launch shape by timing whole replays: 256-lane workgroups at the
+ * occupancy maximum and one below it, 512- and 1024-lane workgroups at theirs. The next ng_cohort_brandes
+ * recaptures with the chosen shape. */
 ng_status ng_cohort_autotune(ng_cohort* c, uint32_t n_sample);
 /* Stats of the last cohort replay (kernel_ms covers the whole replay). */
 ng_status ng_cohort_last_stats(const ng_cohort* c, ng_call_stats* out);

@@ -111,6 +111,7 @@ struct ng_cohort {
     uint32_t total = 0;                // sum of member sizes = number of (graph, source) items
     std::vector<uint32_t> prefix;      // [G+1] item / node prefix per graph
     std::vector<double> delta;         // per graph (0: the graph's tuned or heuristic width)
+    int block = kBlock;                // lanes per workgroup
     int bpc = 0;                       // resident workgroups per CU (0: occupancy maximum)
     uint32_t slots = 0;
     uint64_t ws_generation = 0;        // context workspace generation the capture was made against
@@ -1031,8 +1032,8 @@ ng_status cohort_build(ng_cohort* c) {
         gtab[gi].lend = lend;
     }
     // slots
-    const int maps = lds_maps_for(c->dk, MODE_BC, c->n_max, kBlock);
-    const int bpc_max = batch_blocks_per_cu(c->dk, MODE_BC, c->n_max, maps, kBlock);
+    const int maps = lds_maps_for(c->dk, MODE_BC, c->n_max, c->block);
+    const int bpc_max = batch_blocks_per_cu(c->dk, MODE_BC, c->n_max, maps, c->block);
     const int bpc = c->bpc > 0 ? std::min(c->bpc, bpc_max) : bpc_max;
     uint64_t slots = static_cast<uint64_t>(bpc) * static_cast<uint64_t>(ctx->cus);
     slots = std::min<uint64_t>(slots, std::max<uint32_t>(c->total, 1));
@@ -1099,6 +1100,7 @@ ng_status cohort_build(ng_cohort* c) {
     la.dist_out = nullptr;
     la.dist_out_kind = 0;
     la.maps = maps;
+    la.block = c->block;
     la.gtab = c->d_gtab;
     la.n_graphs = G;
     la.node_prefix = c->d_prefix;
@@ -1157,7 +1159,7 @@ ng_status cohort_replay(ng_cohort* c) {
     l.kernel_ms = ms;
     l.launches = 1;
     l.slots = c->slots;
-    l.block = kBlock;
+    l.block = static_cast<uint32_t>(c->block);
     l.delta = 0.0;
     l.arcs_reached = ctrs[0];
     l.flags = err;
@@ -1227,26 +1229,31 @@ ng_status ng_cohort_autotune(ng_cohort* c, uint32_t n_sample) {
         if (st != NG_OK) return st;
         c->delta[gi] = d;
     }
-    // resident workgroups per CU: the occupancy maximum and two lower values, one whole replay each
-    c->bpc = 0;
-    c->built = false;
-    ng_status st = cohort_replay(c);
-    if (st != NG_OK) return st;
-    const uint32_t max_slots = c->slots;
-    const int bpc_max = std::max(1, static_cast<int>(max_slots / std::max(1, c->ctx->cus)));
-    int best = 0;
-    double best_ms = c->last.kernel_ms;
-    for (int b = bpc_max - 1; b >= std::max(1, bpc_max - 2); --b) {
-        c->bpc = b;
+    // launch shape, one whole replay each: 256-lane workgroups at the occupancy maximum and one below it,
+    // 512- and 1024-lane workgroups at theirs
+    int best_block = kBlock, best_bpc = 0;
+    double best_ms = std::numeric_limits<double>::infinity();
+    uint32_t slots_at_max = 0;
+    for (int cand = 0; cand < 4; ++cand) {
+        c->block = cand < 2 ? 256 : (cand == 2 ? 512 : 1024);
+        c->bpc = 0;
+        if (cand == 1) {
+            const int at_max = static_cast<int>(slots_at_max / std::max(1u, static_cast<uint32_t>(c->ctx->cus)));
+            if (at_max <= 1) continue;
+            c->bpc = at_max - 1;
+        }
         c->built = false;
-        st = cohort_replay(c);
+        ng_status st = cohort_replay(c);
         if (st != NG_OK) return st;
+        if (cand == 0) slots_at_max = c->slots;
         if (c->last.kernel_ms < best_ms) {
             best_ms = c->last.kernel_ms;
-            best = b;
+            best_block = c->block;
+            best_bpc = c->bpc;
         }
     }
-    c->bpc = best;
+    c->block = best_block;
+    c->bpc = best_bpc;
     c->built = false;
     return NG_OK;
 }
