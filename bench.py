@@ -283,7 +283,9 @@ def main():
     ap.add_argument("--batch", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--delta", type=float, default=0.0, help="bucket width (0: autotune)")
+    ap.add_argument("--delta", type=float, default=0.0, help="bucket width (0: autotune width and lanes per source)")
+    ap.add_argument("--block", type=int, default=0, choices=[0, 256, 512, 1024],
+                    help="lanes per source workgroup when --delta is given (0: the library default)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (a known-size kernel for scaling profiler byte counters)")
     args = ap.parse_args()
@@ -292,6 +294,8 @@ def main():
               "one final reduction of the betweenness vector; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    if args.block:
+        os.environ["NG_BLOCK_SIZE"] = str(args.block)  # read by the library for graphs without a tuned shape
     import neurograph_amd as ng
 
     if args.workload == "cohort64":
