@@ -102,7 +102,7 @@ ng_status ng_copy_calibration(ng_context* ctx, uint64_t bytes, double* ms_out);
  * out_offsets[n]=m), out_targets[m] < n, out_weights[m] of `wkind`. Replaces the per-rank RMA windows
  * and remote adjacency cache: DistributedGraph::construct_graph (DistributedGraph.cpp:46-127),
  * get_out_arcs (DistributedGraph.h:275-311). Parallel arcs and self-loops are kept (they are arcs of
- * the reference graph too); m must be < 2^32 (arc_id_type, Types.h:27). */
+ * the reference graph too); m must be < 2^32 (arc_id_type, Types.h:27); real-valued weights must be finite. */
 ng_status ng_graph_create(ng_context* ctx, uint32_t n, uint64_t m, const uint64_t* out_offsets,
                           const uint32_t* out_targets, const void* out_weights, ng_weight_kind wkind,
                           ng_graph** out);

@@ -696,6 +696,11 @@ ng_status ng_graph_create(ng_context* ctx, uint32_t n, uint64_t m, const uint64_
     g->w.resize(wb * m);
     if (m) std::memcpy(g->w.data(), out_weights, wb * m);
     for (uint64_t a = 0; a < m; ++a) {
+        if (!std::isfinite(weight_at(g.get(), a))) {
+            return fail(NG_EINVAL, "the weight of arc %llu is not a finite number", (unsigned long long)a);
+        }
+    }
+    for (uint64_t a = 0; a < m; ++a) {
         if (weight_at(g.get(), a) < 0.0) {
             g->has_negative = true;
             break;
