@@ -897,29 +897,50 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
             const uint32_t nlog = S.cnt[3];
             c_logged += nlog;
             // arc e into t is tight when its candidate c equals the final distance of t
+            uint32_t n_tight = 0;
             auto mark = [&](uint32_t t, uint32_t e, B c) {
                 if (c == ldc(DP(t))) {
                     wg_or(&bits[e >> 5], 1u << (e & 31));
-                    wg_add(&R[t].indeg, 1u);
+                    ++n_tight;
                 }
             };
-            if (nlog <= a.logcap) {
-                for (uint32_t i = tid; i < nlog; i += BS) {
-                    const LogEntry r = log[i];
-                    mark(r.t, r.e, static_cast<B>(r.nd));
+            // every candidate (from the log, or from the full rows when the log overflowed) through fn
+            auto each_candidate = [&](auto fn) {
+                if (nlog <= a.logcap) {
+                    for (uint32_t i = tid; i < nlog; i += BS) {
+                        const LogEntry r = log[i];
+                        fn(r.t, r.e, static_cast<B>(r.nd));
+                    }
+                } else {
+                    expand<BS>(
+                        Sl, nS, S,
+                        [&](uint32_t v, B& dv, uint32_t& beg, uint32_t& cnt) {
+                            dv = 0;
+                            beg = off[v];
+                            cnt = off[v + 1] - beg;
+                        },
+                        [&](uint32_t v, B, uint32_t e) { fn(tgt[e], e, T::add(ldc(DP(v)), len[e])); });
                 }
-            } else {
-                if (tid == 0) atomicOr(a.err, ERR_LOG_OVERFLOW);
-                expand<BS>(
-                    Sl, nS, S,
-                    [&](uint32_t v, B& dv, uint32_t& beg, uint32_t& cnt) {
-                        dv = 0;
-                        beg = off[v];
-                        cnt = off[v + 1] - beg;
-                    },
-                    [&](uint32_t v, B, uint32_t e) { mark(tgt[e], e, T::add(ldc(DP(v)), len[e])); });
+            };
+            if (nlog > a.logcap && tid == 0) atomicOr(a.err, ERR_LOG_OVERFLOW);
+            each_candidate(mark);
+            {
+                auto add32 = [](uint32_t x, uint32_t y) { return x + y; };
+                n_tight = wave_reduce(n_tight, add32);
+                if (lane_id() == 0) atomicAdd(&S.cnt[6], n_tight);
             }
             bsync();
+            // Every reached node but the source has at least one tight in-arc. With exactly one each (the rule
+            // for continuous weights) the tight arcs form a tree: every sigma is 1, the sigma layers need no
+            // in-degrees or sums, and the dependency term is 1 + delta(t) (the general formula with sigma = 1,
+            // bit for bit). Otherwise a second pass counts the tight in-degrees for the general layers.
+            const bool tree = S.cnt[6] + 1u == nS;
+            if (!tree) {
+                each_candidate([&](uint32_t t, uint32_t e, B c) {
+                    if (c == ldc(DP(t))) wg_add(&R[t].indeg, 1u);
+                });
+                bsync();
+            }
             NG_TOCK(t_verify, 4);
             NG_TICK(t_kahn);
             // ---- Kahn layers over the tight DAG: sigma and a topological order
@@ -935,7 +956,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
                 ++nl;
                 for (uint32_t i = lb + tid; i < le; i += BS) {
                     const uint32_t u = order[i];
-                    const double su = ldc(&R[u].sigma);
+                    const double su = tree ? 1.0 : ldc(&R[u].sigma);
                     const uint32_t b = off[u], e_end = off[u + 1];
                     if (b < e_end) {
                         for (uint32_t wi = b >> 5; wi <= (e_end - 1) >> 5; ++wi) {
@@ -944,8 +965,12 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
                                 const uint32_t e = (wi << 5) + static_cast<uint32_t>(__ffs(w) - 1);
                                 w &= w - 1;
                                 const uint32_t t = tgt[e];
-                                wg_add(&R[t].sigma, su);
-                                if (wg_sub(&R[t].indeg, 1u) == 1u) {
+                                bool released = true;  // a tree node's only tight in-arc releases it
+                                if (!tree) {
+                                    wg_add(&R[t].sigma, su);
+                                    released = wg_sub(&R[t].indeg, 1u) == 1u;
+                                }
+                                if (released) {
                                     const uint32_t pos = atomicAdd(&S.cnt[4], 1u);
                                     if (pos < n) order[pos] = t;
                                 }
@@ -978,7 +1003,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
                 const uint32_t l0 = layers[ly], l1 = layers[ly + 1];
                 for (uint32_t i = l0 + tid; i < l1; i += BS) {
                     const uint32_t u = order[i];
-                    const double su = ldc(&R[u].sigma);
+                    const double su = tree ? 1.0 : ldc(&R[u].sigma);
                     double acc1 = 0.0, acc2 = 0.0;
                     const uint32_t b = off[u], e_end = off[u + 1];
                     if (b < e_end) {
@@ -990,7 +1015,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
                                 const uint32_t e = (wi << 5) + static_cast<uint32_t>(__ffs(w) - 1);
                                 w &= w - 1;
                                 const uint32_t t = tgt[e];
-                                const double st = ldc(&R[t].sigma);
+                                const double st = tree ? 1.0 : ldc(&R[t].sigma);
                                 const double term1 = su * ((1.0 + ldc(&R[t].delta1)) / st);
                                 acc1 += term1;
                                 if constexpr ((MODE & MODE_ARC1) != 0) unsafeAtomicAdd(&a.grad1[perm[e]], term1);
