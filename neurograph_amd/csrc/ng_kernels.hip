@@ -935,6 +935,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
             // in-degrees or sums, and the dependency term is 1 + delta(t) (the general formula with sigma = 1,
             // bit for bit). Otherwise a second pass counts the tight in-degrees for the general layers.
             const bool tree = S.cnt[6] + 1u == nS;
+            // Brandes on a tree pushes dependencies to parents: every delta is a subtree size, an integer, so the
+            // atomic sums are exact in any order. The sigma layers record each node's parent and clear the bitmap.
+            const bool push_dep = MODE == MODE_BC && tree;
             if (!tree) {
                 each_candidate([&](uint32_t t, uint32_t e, B c) {
                     if (c == ldc(DP(t))) wg_add(&R[t].indeg, 1u);
@@ -961,10 +964,12 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
                     if (b < e_end) {
                         for (uint32_t wi = b >> 5; wi <= (e_end - 1) >> 5; ++wi) {
                             uint32_t w = ldc(&bits[wi]) & range_mask(wi, b, e_end);
+                            if (push_dep && w != 0u) wg_and(&bits[wi], ~w);
                             while (w) {
                                 const uint32_t e = (wi << 5) + static_cast<uint32_t>(__ffs(w) - 1);
                                 w &= w - 1;
                                 const uint32_t t = tgt[e];
+                                if (push_dep) stc(&R[t].spare, u);
                                 bool released = true;  // a tree node's only tight in-arc releases it
                                 if (!tree) {
                                     wg_add(&R[t].sigma, su);
@@ -999,42 +1004,55 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
             NG_TOCK(t_kahn, 5);
             NG_TICK(t_back);
             // ---- reverse layers: dependencies, per-node and per-arc accumulation, bitmap cleared
-            for (int ly = static_cast<int>(nl) - 1; ly >= 0; --ly) {
-                const uint32_t l0 = layers[ly], l1 = layers[ly + 1];
-                for (uint32_t i = l0 + tid; i < l1; i += BS) {
-                    const uint32_t u = order[i];
-                    const double su = tree ? 1.0 : ldc(&R[u].sigma);
-                    double acc1 = 0.0, acc2 = 0.0;
-                    const uint32_t b = off[u], e_end = off[u + 1];
-                    if (b < e_end) {
-                        for (uint32_t wi = b >> 5; wi <= (e_end - 1) >> 5; ++wi) {
-                            uint32_t w = ldc(&bits[wi]) & range_mask(wi, b, e_end);
-                            if (w == 0u) continue;
-                            wg_and(&bits[wi], ~w);
-                            while (w) {
-                                const uint32_t e = (wi << 5) + static_cast<uint32_t>(__ffs(w) - 1);
-                                w &= w - 1;
-                                const uint32_t t = tgt[e];
-                                const double st = tree ? 1.0 : ldc(&R[t].sigma);
-                                const double term1 = su * ((1.0 + ldc(&R[t].delta1)) / st);
-                                acc1 += term1;
-                                if constexpr ((MODE & MODE_ARC1) != 0) unsafeAtomicAdd(&a.grad1[perm[e]], term1);
-                                if constexpr ((MODE & MODE_ARC2) != 0) {
-                                    const double dt = T::val(ldc(DP(t)));
-                                    const double term2 = su * ((1.0 / (dt * dt) + ldc(&delta2[t])) / st);
-                                    acc2 += term2;
-                                    unsafeAtomicAdd(&a.grad2[perm[e]], term2);
+            if (push_dep) {
+                for (int ly = static_cast<int>(nl) - 1; ly >= 1; --ly) {
+                    const uint32_t l0 = layers[ly], l1 = layers[ly + 1];
+                    for (uint32_t i = l0 + tid; i < l1; i += BS) {
+                        const uint32_t t = order[i];
+                        const double d = ldc(&R[t].delta1);
+                        R[t].bc += d;
+                        if (ly > 1) wg_add(&R[ldc(&R[t].spare)].delta1, 1.0 + d);
+                    }
+                    bsync();
+                }
+            } else {
+                for (int ly = static_cast<int>(nl) - 1; ly >= 0; --ly) {
+                    const uint32_t l0 = layers[ly], l1 = layers[ly + 1];
+                    for (uint32_t i = l0 + tid; i < l1; i += BS) {
+                        const uint32_t u = order[i];
+                        const double su = tree ? 1.0 : ldc(&R[u].sigma);
+                        double acc1 = 0.0, acc2 = 0.0;
+                        const uint32_t b = off[u], e_end = off[u + 1];
+                        if (b < e_end) {
+                            for (uint32_t wi = b >> 5; wi <= (e_end - 1) >> 5; ++wi) {
+                                uint32_t w = ldc(&bits[wi]) & range_mask(wi, b, e_end);
+                                if (w == 0u) continue;
+                                wg_and(&bits[wi], ~w);
+                                while (w) {
+                                    const uint32_t e = (wi << 5) + static_cast<uint32_t>(__ffs(w) - 1);
+                                    w &= w - 1;
+                                    const uint32_t t = tgt[e];
+                                    const double st = tree ? 1.0 : ldc(&R[t].sigma);
+                                    const double term1 = su * ((1.0 + ldc(&R[t].delta1)) / st);
+                                    acc1 += term1;
+                                    if constexpr ((MODE & MODE_ARC1) != 0) unsafeAtomicAdd(&a.grad1[perm[e]], term1);
+                                    if constexpr ((MODE & MODE_ARC2) != 0) {
+                                        const double dt = T::val(ldc(DP(t)));
+                                        const double term2 = su * ((1.0 / (dt * dt) + ldc(&delta2[t])) / st);
+                                        acc2 += term2;
+                                        unsafeAtomicAdd(&a.grad2[perm[e]], term2);
+                                    }
                                 }
                             }
                         }
+                        if (ly > 0) {
+                            stc(&R[u].delta1, acc1);
+                            if constexpr ((MODE & MODE_ARC2) != 0) stc(&delta2[u], acc2);
+                            if constexpr ((MODE & MODE_BC) != 0) R[u].bc += acc1;
+                        }
                     }
-                    if (ly > 0) {
-                        stc(&R[u].delta1, acc1);
-                        if constexpr ((MODE & MODE_ARC2) != 0) stc(&delta2[u], acc2);
-                        if constexpr ((MODE & MODE_BC) != 0) R[u].bc += acc1;
-                    }
+                    bsync();
                 }
-                bsync();
             }
             NG_TOCK(t_back, 6);
         }
