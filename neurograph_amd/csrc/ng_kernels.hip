@@ -603,7 +603,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
         auto apply = [&](bool ok, uint32_t v, B dv, uint32_t e, uint32_t t, L l, bool final_t, B cur, uint32_t nph,
                          bool log_it_if_close) {
             const B nd = T::add(dv, l);
-            bool pf = false, pp = false;
+            bool pf = false, pp = false, parked = false;
             if (ok && !final_t && nd < cur) {
                 const B old = wg_min(DP(t), nd);
                 if (nd < old) {
@@ -611,7 +611,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
                     if (nd < hi) {
                         pf = use_fb ? fb_first(t) : wg_exch(&fstamp[t], nph) != nph;
                     } else {
-                        atomicMin(&S.runmin, nd);
+                        parked = true;
                         // first parking of t in this source (its first reach: every later reach finds a finite
                         // distance) always fits, the pile keeps n entries in reserve; a re-parking with the
                         // improved value only while the reserve is untouched, else the next bucket advance
@@ -628,6 +628,11 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
             }
             push(pf, t, Fn, &S.cnt[0], n);
             push_entry(pp, nd, t, Pc, &S.cnt[1], pcap);
+            if (__ballot(parked) != 0ull) {  // the smallest parked value, one LDS atomic per wave
+                auto minb = [](B x, B y) { return x < y ? x : y; };
+                const B m = wave_reduce(parked ? nd : T::kInf, minb);
+                if (lane_id() == 0) atomicMin(&S.runmin, m);
+            }
             if constexpr (MODE != MODE_SSSP) {
                 const bool lg = ok && log_it_if_close && !final_t && nd <= cur;
                 const unsigned long long mask = __ballot(lg);
@@ -788,28 +793,44 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
             }
             bsync();
             // split the pile: entries of settled nodes are dropped, values below hi join the frontier
-            // (duplicates removed by the frontier stamp), the rest is carried over with its value
-            for (uint32_t base = 0; base < nP; base += BS) {
-                const uint32_t i = base + tid;
-                bool tf = false, tp = false;
-                uint32_t t = 0;
-                B d = 0;
-                if (i < nP) {
-                    const PileEntry<B> pe = Pc[i];
-                    t = pe.node;
-                    const bool settled = use_bm ? ((bm[t >> 5] >> (t & 31)) & 1u) != 0u : ldc(&sstamp[t]) == gen;
-                    if (!settled) {
-                        d = exact ? ldc(DP(t)) : pe.val;
-                        if (d < hi) {
-                            tf = use_fb ? fb_first(t) : wg_exch(&fstamp[t], nph) != nph;
-                        } else {
-                            atomicMin(&S.runmin, d);
-                            tp = true;
+            // (duplicates removed by the frontier stamp), the rest is carried over with its value. Each lane
+            // has kSplit entries in flight; the minimum carried value is reduced per wave (one LDS atomic).
+            constexpr uint32_t kSplit = 4;
+            B lmin = T::kInf;
+            for (uint32_t base = 0; base < nP; base += kSplit * BS) {
+                PileEntry<B> pe[kSplit];
+#pragma unroll
+                for (uint32_t j = 0; j < kSplit; ++j) {
+                    const uint32_t i = base + j * BS + tid;
+                    if (i < nP) pe[j] = Pc[i];
+                }
+#pragma unroll
+                for (uint32_t j = 0; j < kSplit; ++j) {
+                    const uint32_t i = base + j * BS + tid;
+                    bool tf = false, tp = false;
+                    uint32_t t = 0;
+                    B d = 0;
+                    if (i < nP) {
+                        t = pe[j].node;
+                        const bool settled = use_bm ? ((bm[t >> 5] >> (t & 31)) & 1u) != 0u : ldc(&sstamp[t]) == gen;
+                        if (!settled) {
+                            d = exact ? ldc(DP(t)) : pe[j].val;
+                            if (d < hi) {
+                                tf = use_fb ? fb_first(t) : wg_exch(&fstamp[t], nph) != nph;
+                            } else {
+                                lmin = d < lmin ? d : lmin;
+                                tp = true;
+                            }
                         }
                     }
+                    push(tf, t, Fn, &S.cnt[0], n);
+                    push_entry(tp, d, t, Pn, &S.cnt[1], pcap);
                 }
-                push(tf, t, Fn, &S.cnt[0], n);
-                push_entry(tp, d, t, Pn, &S.cnt[1], pcap);
+            }
+            {
+                auto minb = [](B x, B y) { return x < y ? x : y; };
+                lmin = wave_reduce(lmin, minb);
+                if (lane_id() == 0 && lmin != T::kInf) atomicMin(&S.runmin, lmin);
             }
             bsync();
             nF = clampn(S.cnt[0]);
