@@ -131,7 +131,7 @@ struct LaunchArgs {
     Workspace* ws;
     void* dist_out;           // device, optional
     int dist_out_kind;        // ng_dist_kind
-    int maps;                 // extra LDS bitmaps: MAP_SORTED (settle passes in increasing id), MAP_FRONTIER
+    int maps;                 // extra LDS maps: MAP_SORTED (settle passes in increasing id), MAP_FRONTIER, MAP_BOUND
     int block = kBlock;       // lanes per workgroup: 256, 512 or 1024
     // cohort launch (gtab != nullptr; view, sources and n_sources unused; n = the largest member's n)
     const CohortGraph* gtab = nullptr;
@@ -162,8 +162,9 @@ hipError_t launch_distance_histogram(int dist_kind, const void* dist, const uint
 hipError_t launch_fill_u32(uint32_t* p, uint64_t count, uint32_t value, hipStream_t stream);
 hipError_t launch_fill_f64(double* p, uint64_t count, double value, hipStream_t stream);
 hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t stream);  // bytes % 16 == 0
-// LDS bitmaps beyond the finalized-node one (bit flags of LaunchArgs::maps).
-enum : int { MAP_SORTED = 1, MAP_FRONTIER = 2 };
+// Extra LDS maps beyond the finalized-node bitmap (bit flags of LaunchArgs::maps): bitmaps of each bucket's
+// settled nodes and of the next frontier, and 16-bit distance bounds per node.
+enum : int { MAP_SORTED = 1, MAP_FRONTIER = 2, MAP_BOUND = 4 };
 // Resident workgroups per CU of the batched kernel for (dk, mode, extra LDS bitmaps, lanes per workgroup).
 int batch_blocks_per_cu(DKind dk, int mode, uint32_t n, int maps, int block);
 

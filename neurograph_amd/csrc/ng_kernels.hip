@@ -327,13 +327,40 @@ __device__ __forceinline__ void expand_relax(const uint32_t* list, uint32_t coun
             }
 #pragma unroll
             for (int j = 0; j < U; ++j) {
-                if (pok[j]) probe(pt[j], pl[j], pfin[j], pcur[j]);
+                if (pok[j]) probe(pt[j], pl[j], pdv[j], pfin[j], pcur[j]);
             }
 #pragma unroll
             for (int j = 0; j < U; ++j) apply(pok[j], pv[j], pdv[j], pe[j], pt[j], pl[j], pfin[j], pcur[j]);
         }
         __syncthreads();
     }
+}
+
+// Per-node distance bounds in LDS: the distance rounded up to a bfloat16 (the upper 16 bits of an f32), 2 bytes
+// per node and scale-free. A stored bound is never below the node's current distance: distances only fall, and
+// each improvement stores the bound of its own value, so a late store of an earlier (larger) value is still an
+// upper bound. A candidate above a node's bound can neither improve its distance nor be tight, so its global
+// distance probe is skipped. kBoundInf (+inf) is the bound of an unreached node.
+constexpr uint16_t kBoundInf = 0x7F80u;
+
+template <class T>
+__device__ __forceinline__ uint16_t bound16(typename T::B b) {
+    if constexpr (T::kInt && sizeof(typename T::B) == 8) {
+        if (b >= (1ull << 53)) return kBoundInf;  // beyond exact doubles: no bound
+    }
+    const double d = T::val(b);
+    const float f = static_cast<float>(d);
+    uint32_t bits = __float_as_uint(f);
+    if (static_cast<double>(f) < d) ++bits;  // round up to the next float (d >= 0)
+    bits = (bits + 0xFFFFu) & 0xFFFF0000u;   // round up to a bfloat16
+    return static_cast<uint16_t>(bits >> 16);
+}
+
+// Candidate c is above bound bd. Comparing in double is exact for every bound; a candidate that rounds onto
+// the bound is not above it, which only costs a probe.
+template <class T>
+__device__ __forceinline__ bool above_bound(typename T::B c, uint16_t bd) {
+    return T::val(c) > static_cast<double>(__uint_as_float(static_cast<uint32_t>(bd) << 16));
 }
 
 template <class D>
@@ -372,6 +399,7 @@ struct KArgs {
     double* flush;
     const uint64_t* flush_base;
     uint32_t bm_words;  // words of the LDS bitmap of finalized nodes (0: no bitmap)
+    uint32_t bd_words;  // words of the LDS distance bounds after the bitmaps (0: no bounds)
     uint32_t maps;      // extra LDS bitmaps: bit 0 orders each bucket's settled nodes by id, bit 1 marks the next frontier
     double* grad1;
     double* grad2;
@@ -454,6 +482,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
     // phase can be pushed once more.
     const bool use_fb = use_bm && (a.maps & 2u) != 0;
     uint32_t* const fb = bm + a.bm_words * (sorted ? 2u : 1u);
+    // With `use_bd`, 16-bit distance bounds (bound16) of every node follow the bitmaps.
+    const bool use_bd = use_bm && a.bd_words != 0;
+    uint16_t* const fl = reinterpret_cast<uint16_t*>(bm + a.bm_words * (1u + (sorted ? 1u : 0u) + (use_fb ? 1u : 0u)));
     auto fb_first = [fb](uint32_t t) -> bool {
         const uint32_t bit = 1u << (t & 31);
         return (atomicOr(&fb[t >> 5], bit) & bit) == 0u;
@@ -561,11 +592,18 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
             for (uint32_t w = tid; w < bmw; w += BS) bn[w] = 0u;
         if (use_fb)
             for (uint32_t w = tid; w < bmw; w += BS) fb[w] = 0u;
+        if (use_bd) {
+            uint32_t* const flw = reinterpret_cast<uint32_t*>(fl);
+            const uint32_t bdw = (n + 1) / 2;
+            for (uint32_t w = tid; w < bdw; w += BS) flw[w] = (static_cast<uint32_t>(kBoundInf) << 16) | kBoundInf;
+        }
+        bsync();
         if (tid == 0) {
 #pragma unroll
             for (int k = 0; k < 8; ++k) S.cnt[k] = 0;
             S.runmin = T::kInf;
             stc(DP(s), B(0));
+            if (use_bd) fl[s] = 0;
             Fa[0] = s;
         }
         uint32_t* Fc = Fa;
@@ -592,9 +630,11 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
         };
         bsync();
 
-        // distance probe of arc target t: skipped (fin) for nodes finalized in earlier buckets
-        auto probe = [&](uint32_t t, L l, bool& fin, B& cur) {
+        // distance probe of arc target t: skipped (fin) for heavy arcs into registered nodes, and for candidates
+        // above the target's bound
+        auto probe = [&](uint32_t t, L l, B dv, bool& fin, B& cur) {
             fin = use_bm && skip_ok && !(l < delta) && ((bm[t >> 5] >> (t & 31)) & 1u);
+            if (use_bd && !fin) fin = above_bound<T>(T::add(dv, l), fl[t]);
             cur = fin ? B(0) : ldc(DP(t));
             c_probe += fin ? 0u : 1u;
         };
@@ -608,6 +648,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
                 const B old = wg_min(DP(t), nd);
                 if (nd < old) {
                     ++c_improve;
+                    if (use_bd) fl[t] = bound16<T>(nd);
                     if (nd < hi) {
                         pf = use_fb ? fb_first(t) : wg_exch(&fstamp[t], nph) != nph;
                     } else {
@@ -919,8 +960,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
             c_logged += nlog;
             // arc e into t is tight when its candidate c equals the final distance of t
             uint32_t n_tight = 0;
+            auto tight = [&](uint32_t t, B c) -> bool { return !(use_bd && above_bound<T>(c, fl[t])) && c == ldc(DP(t)); };
             auto mark = [&](uint32_t t, uint32_t e, B c) {
-                if (c == ldc(DP(t))) {
+                if (tight(t, c)) {
                     wg_or(&bits[e >> 5], 1u << (e & 31));
                     ++n_tight;
                 }
@@ -961,7 +1003,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
             const bool push_dep = MODE == MODE_BC && tree;
             if (!tree) {
                 each_candidate([&](uint32_t t, uint32_t e, B c) {
-                    if (c == ldc(DP(t))) wg_add(&R[t].indeg, 1u);
+                    if (tight(t, c)) wg_add(&R[t].indeg, 1u);
                 });
                 bsync();
             }
@@ -1172,13 +1214,34 @@ hipError_t fill(T* p, uint64_t count, T value, hipStream_t stream) {
     return hipGetLastError();
 }
 
-// LDS bitmaps of the batched kernel: the finalized-node bitmap plus one per bit of `maps`.
-uint32_t lds_maps(int maps) { return 1u + (maps & 1 ? 1u : 0u) + (maps & 2 ? 1u : 0u); }
+// LDS bitmaps of the batched kernel: the finalized-node bitmap plus one per bit of `maps` (MAP_SORTED,
+// MAP_FRONTIER).
+uint32_t lds_maps(int maps) { return 1u + (maps & MAP_SORTED ? 1u : 0u) + (maps & MAP_FRONTIER ? 1u : 0u); }
 
-// Words of each LDS bitmap, or 0 when the bitmaps would not fit next to the static LDS.
-uint32_t bitmap_words(uint32_t n, int maps) {
+// Dynamic LDS of the batched kernel for n nodes: the bitmaps (none when they exceed 96 KB), then the 16-bit
+// distance bounds with MAP_BOUND when everything stays within 136 KB (the static LDS takes at most ~21 KB
+// of the 160 KB of a CU).
+struct LdsPlan {
+    uint32_t bm_words = 0;
+    uint32_t bd_words = 0;
+    int maps = 0;
+    size_t bytes = 0;
+};
+LdsPlan lds_plan(uint32_t n, int maps) {
+    LdsPlan p;
     const uint32_t w = (n + 31) / 32;
-    return static_cast<uint64_t>(w) * 4 * lds_maps(maps) <= 96 * 1024 ? w : 0u;
+    const size_t bm_bytes = static_cast<size_t>(w) * 4 * lds_maps(maps);
+    if (bm_bytes > 96 * 1024) return p;
+    p.bm_words = w;
+    p.maps = maps & (MAP_SORTED | MAP_FRONTIER);
+    p.bytes = bm_bytes;
+    const uint32_t bw = (n + 1) / 2;
+    if ((maps & MAP_BOUND) != 0 && bm_bytes + static_cast<size_t>(bw) * 4 <= 136 * 1024) {
+        p.bd_words = bw;
+        p.maps |= MAP_BOUND;
+        p.bytes += static_cast<size_t>(bw) * 4;
+    }
+    return p;
 }
 
 template <class D, int MODE, int BS>
@@ -1234,10 +1297,11 @@ hipError_t launch_typed(const LaunchArgs& la, hipStream_t stream) {
     k.err = ws.err;
     k.dist_out = la.dist_out;
     k.dist_out_kind = la.dist_out_kind;
-    k.bm_words = bitmap_words(la.n, la.maps);
-    k.maps = k.bm_words ? static_cast<uint32_t>(la.maps) : 0u;
-    hipLaunchKernelGGL((batch_kernel<D, MODE, BS>), dim3(la.grid), dim3(BS), k.bm_words * 4 * lds_maps(static_cast<int>(k.maps)),
-                       stream, k);
+    const LdsPlan lp = lds_plan(la.n, la.maps);
+    k.bm_words = lp.bm_words;
+    k.bd_words = lp.bd_words;
+    k.maps = static_cast<uint32_t>(lp.maps);
+    hipLaunchKernelGGL((batch_kernel<D, MODE, BS>), dim3(la.grid), dim3(BS), lp.bytes, stream, k);
     return hipGetLastError();
 }
 
@@ -1279,8 +1343,7 @@ int blocks_per_cu_bs(int mode, size_t dyn) {
 
 template <class D>
 int blocks_per_cu_mode(int mode, uint32_t n, int maps, int block) {
-    const uint32_t bw = bitmap_words(n, maps);
-    const size_t dyn = static_cast<size_t>(bw) * 4 * (bw ? lds_maps(maps) : 0u);
+    const size_t dyn = lds_plan(n, maps).bytes;
     switch (block) {
     case 512: return blocks_per_cu_bs<D, 512>(mode, dyn);
     case 1024: return blocks_per_cu_bs<D, 1024>(mode, dyn);

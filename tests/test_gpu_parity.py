@@ -249,22 +249,32 @@ def test_small_pile_forces_exact_bucket_advance(ctx, monkeypatch):
         np.testing.assert_allclose(bc, orc.brandes(c, sources=srcs)["bc"], rtol=1e-9, atol=1e-9)
 
 
-@pytest.mark.parametrize("maps", ["0", "1", "2", "3"])
+@pytest.mark.parametrize("maps", [str(k) for k in range(8)])
 def test_lds_bitmap_variants(ctx, monkeypatch, maps):
-    """Every combination of the optional LDS bitmaps (settle passes in discovery order or in increasing id;
-    frontier de-duplication by global stamps or by an LDS bitmap) gives the oracle's distances bit-exactly and
-    its betweenness within 1e-9."""
+    """Every combination of the optional LDS maps (settle passes in discovery order or in increasing id;
+    frontier de-duplication by global stamps or by an LDS bitmap; probes with or without the 16-bit distance
+    bounds) gives the oracle's distances bit-exactly and its betweenness within 1e-9, for integer, FP32 and FP64
+    weights and both length kinds (|w| and 1/|w|)."""
     monkeypatch.setenv("NG_LDS_MAPS", maps)
-    for wdtype, seed in ((np.int32, 5), (np.float64, 6)):
-        c = rand_graph(4000, 40000, seed, wdtype=wdtype, wlo=1, whi=9)
+    # int32 weights up to 2^31 - 1 take the 64-bit integer distances, whose bounds round
+    for wdtype, seed, wlo, whi in ((np.int32, 5, 1, 9), (np.int32, 9, 1 << 20, 2**31 - 1), (np.float32, 7, 1, 9),
+                                   (np.float64, 6, 1, 9)):
+        c = rand_graph(4000, 40000, seed, wdtype=wdtype, wlo=wlo, whi=whi)
         g = dev_graph(ctx, c)
         srcs = np.arange(0, 4000, 29)
-        ref = np.stack([orc.sssp(c, int(s))[0] for s in srcs])
-        for delta in (0.0, 0.7, 5.0):
-            _, dist = g.sssp(srcs, delta=delta, want_dist=True)
-            np.testing.assert_array_equal(dist, ref)
+        for lkind in (orc.L_ABS, orc.L_INVERSE):
+            ref = np.stack([orc.sssp(c, int(s), lkind)[0] for s in srcs])
+            scale = (whi if lkind == orc.L_ABS else 1.0 / whi) / 9.0
+            for delta in (0.0, 0.7, 5.0):
+                _, dist = g.sssp(srcs, lkind=lkind, delta=delta * scale, want_dist=True)
+                np.testing.assert_array_equal(dist, ref)
         bc, summ = g.brandes(srcs, summaries=True)
         np.testing.assert_allclose(bc, orc.brandes(c, sources=srcs)["bc"], rtol=1e-9, atol=1e-9)
+    # a graph of unit weights ties everywhere (sigma > 1, the general dependency layers)
+    c = rand_graph(3000, 30000, 8, wdtype=np.int32, wlo=1, whi=2)
+    g = dev_graph(ctx, c)
+    srcs = np.arange(0, 3000, 37)
+    np.testing.assert_allclose(g.brandes(srcs), orc.brandes(c, sources=srcs)["bc"], rtol=1e-9, atol=1e-9)
 
 
 def test_shared_workspace_across_graphs(ctx):
