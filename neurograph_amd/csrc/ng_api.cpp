@@ -305,17 +305,18 @@ void free_ws(Workspace& w) {
     w.generation = gen + 1;
 }
 
-// Extra LDS maps, each used only when it costs no resident workgroups, the distance bounds first: MAP_BOUND
-// skips the distance probes of candidates above their target's bound; MAP_SORTED makes settle passes expand
-// each bucket's settled nodes in increasing id (measured +5 % on a 50k-node spatial graph, a loss wherever
-// its LDS lowers occupancy); MAP_FRONTIER replaces the global frontier stamps. NG_LDS_MAPS=0..7 overrides
-// the choice (tuning experiments).
-int lds_maps_for(DKind dk, int mode, uint32_t n, int block) {
-    if (const char* e = std::getenv("NG_LDS_MAPS")) return std::atoi(e) & 7;
+// Extra LDS maps, a set used only when all of it fits and it costs no resident workgroups, the distance
+// bounds first: MAP_BOUND skips the distance probes of candidates above their target's bound (74 % of the
+// probes on a 50k-node spatial graph), with 4-bit codes (MAP_BOUND4, when allowed) where 16-bit bounds do
+// not fit; MAP_SORTED makes settle passes expand each bucket's settled nodes in increasing id; MAP_FRONTIER
+// replaces the global frontier stamps. NG_LDS_MAPS=0..15 overrides the choice (tuning experiments).
+int lds_maps_for(DKind dk, int mode, uint32_t n, int block, bool allow4) {
+    const int b = allow4 ? (MAP_BOUND | MAP_BOUND4) : static_cast<int>(MAP_BOUND);
+    if (const char* e = std::getenv("NG_LDS_MAPS")) return std::atoi(e) & (allow4 ? 15 : 7);
     const int base = batch_blocks_per_cu(dk, mode, n, 0, block);
-    for (int maps : {MAP_BOUND | MAP_SORTED | MAP_FRONTIER, MAP_BOUND | MAP_FRONTIER, MAP_BOUND | MAP_SORTED, static_cast<int>(MAP_BOUND),
-                     MAP_SORTED | MAP_FRONTIER, static_cast<int>(MAP_FRONTIER), static_cast<int>(MAP_SORTED)})
-        if (batch_blocks_per_cu(dk, mode, n, maps, block) >= base) return maps;
+    for (int maps : {b | MAP_SORTED | MAP_FRONTIER, b | MAP_FRONTIER, b | MAP_SORTED, b, MAP_SORTED | MAP_FRONTIER,
+                     static_cast<int>(MAP_FRONTIER), static_cast<int>(MAP_SORTED)})
+        if (lds_plan_maps(n, maps) == maps && batch_blocks_per_cu(dk, mode, n, maps, block) >= base) return maps;
     return 0;
 }
 
@@ -453,7 +454,10 @@ struct RunSpec {
     void* dist_out;   // host, optional
     void* dist_dev = nullptr;  // device, optional (n_sources * n elements, kept on the device)
     int block = 0;             // lanes per workgroup (0: tuned or default)
+    bool no_bound4 = false;    // no 4-bit distance codes (the run that estimates their scale)
 };
+
+ng_status estimate_bound_scale(ng_graph* g, int vi, double delta);
 
 // Uploads sources, sizes the workspace, launches the batched kernel once, and synchronizes.
 ng_status run_batch(ng_graph* g, const RunSpec& rs, uint32_t* slots_used, double* used_delta) {
@@ -476,7 +480,11 @@ ng_status run_batch(ng_graph* g, const RunSpec& rs, uint32_t* slots_used, double
     }
     // slots: resident workgroups of this kernel, bounded by the sources and by device memory
     const int block = pick_block(v, mode_idx, rs.block);
-    const int maps = lds_maps_for(v.dk, rs.mode, g->n, block);
+    const int maps = lds_maps_for(v.dk, rs.mode, g->n, block, !rs.no_bound4);
+    if (lds_plan_bound_kind(g->n, maps) == 2 && !(v.bound_inv_q > 0.0)) {
+        st = estimate_bound_scale(g, rs.vi, delta);
+        if (st != NG_OK) return st;
+    }
     const int bpc = batch_blocks_per_cu(v.dk, rs.mode, g->n, maps, block);
     uint64_t slots = static_cast<uint64_t>(bpc) * static_cast<uint64_t>(ctx->cus);
     // NG_MAX_SLOTS caps the concurrent source slots (for tuning sweeps and experiments)
@@ -521,6 +529,7 @@ ng_status run_batch(ng_graph* g, const RunSpec& rs, uint32_t* slots_used, double
     la.dist_out = d_dist_out;
     la.dist_out_kind = dok;
     la.maps = maps;
+    la.bd_inv_q = v.bound_inv_q;
     la.block = block;
     float ms = 0.0f;
     if (rs.n_sources) {
@@ -578,6 +587,26 @@ ng_status check_graph(const ng_graph* g) {
 ng_status copy_summaries(ng_graph* g, uint32_t n_sources, ng_sssp_summary* out) {
     if (!out || !n_sources) return NG_OK;
     NG_HIP(hipMemcpy(out, g->ctx->ws.summaries, sizeof(ng_sssp_summary) * n_sources, hipMemcpyDeviceToHost));
+    return NG_OK;
+}
+
+// Scale of the 4-bit distance codes of a view: 15 codes over 1.1 x the largest distance from 32 evenly spaced
+// sources (one small SSSP launch, once per view). Farther nodes keep code 15 (no bound), so the estimate only
+// affects how many probes are skipped, never the results.
+ng_status estimate_bound_scale(ng_graph* g, int vi, double delta) {
+    const uint32_t k = std::min<uint32_t>(g->n, 32);
+    std::vector<uint32_t> src(k);
+    for (uint32_t i = 0; i < k; ++i) src[i] = static_cast<uint32_t>(static_cast<uint64_t>(i) * g->n / k);
+    RunSpec rs{vi, MODE_SSSP, delta, src.data(), k, nullptr};
+    rs.no_bound4 = true;
+    ng_status st = run_batch(g, rs, nullptr, nullptr);
+    if (st != NG_OK) return st;
+    std::vector<ng_sssp_summary> sm(k);
+    st = copy_summaries(g, k, sm.data());
+    if (st != NG_OK) return st;
+    double mx = 0.0;
+    for (const ng_sssp_summary& r : sm) mx = std::max(mx, r.max_d);
+    g->views[vi].bound_inv_q = mx > 0.0 && std::isfinite(mx) ? 15.0 / (1.1 * mx) : 1.0;
     return NG_OK;
 }
 
@@ -1039,7 +1068,7 @@ ng_status cohort_build(ng_cohort* c) {
         gtab[gi].lend = lend;
     }
     // slots
-    const int maps = lds_maps_for(c->dk, MODE_BC, c->n_max, c->block);
+    const int maps = lds_maps_for(c->dk, MODE_BC, c->n_max, c->block, false);
     const int bpc_max = batch_blocks_per_cu(c->dk, MODE_BC, c->n_max, maps, c->block);
     const int bpc = c->bpc > 0 ? std::min(c->bpc, bpc_max) : bpc_max;
     uint64_t slots = static_cast<uint64_t>(bpc) * static_cast<uint64_t>(ctx->cus);

@@ -37,6 +37,7 @@ struct View {
     double tuned_delta[2] = {0.0, 0.0};  // per mode (0 SSSP, 1 Brandes); 0 = not tuned
     int tuned_block[2] = {0, 0};         // lanes per workgroup per mode; 0 = not tuned
     double lend_delta = -1.0;
+    double bound_inv_q = 0.0;  // scale of the 4-bit distance codes: 15 / (1.1 x largest sampled distance); 0 = not estimated
     uint32_t* lend = nullptr;  // [n] end of the light prefix of each row for lend_delta
 };
 
@@ -131,7 +132,8 @@ struct LaunchArgs {
     Workspace* ws;
     void* dist_out;           // device, optional
     int dist_out_kind;        // ng_dist_kind
-    int maps;                 // extra LDS maps: MAP_SORTED (settle passes in increasing id), MAP_FRONTIER, MAP_BOUND
+    int maps;                 // extra LDS maps: MAP_SORTED (settle passes in increasing id), MAP_FRONTIER, MAP_BOUND(4)
+    double bd_inv_q = 0.0;    // scale of the 4-bit distance codes (MAP_BOUND4)
     int block = kBlock;       // lanes per workgroup: 256, 512 or 1024
     // cohort launch (gtab != nullptr; view, sources and n_sources unused; n = the largest member's n)
     const CohortGraph* gtab = nullptr;
@@ -163,8 +165,13 @@ hipError_t launch_fill_u32(uint32_t* p, uint64_t count, uint32_t value, hipStrea
 hipError_t launch_fill_f64(double* p, uint64_t count, double value, hipStream_t stream);
 hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t stream);  // bytes % 16 == 0
 // Extra LDS maps beyond the finalized-node bitmap (bit flags of LaunchArgs::maps): bitmaps of each bucket's
-// settled nodes and of the next frontier, and 16-bit distance bounds per node.
-enum : int { MAP_SORTED = 1, MAP_FRONTIER = 2, MAP_BOUND = 4 };
+// settled nodes and of the next frontier, 16-bit distance bounds per node, and 4-bit distance codes where the
+// 16-bit bounds do not fit (MAP_BOUND4 needs LaunchArgs::bd_inv_q).
+enum : int { MAP_SORTED = 1, MAP_FRONTIER = 2, MAP_BOUND = 4, MAP_BOUND4 = 8 };
+// The maps of `maps` that fit next to the kernel's static LDS for n nodes, and the bound kind they give
+// (0: none, 1: 16-bit, 2: 4-bit).
+int lds_plan_maps(uint32_t n, int maps);
+int lds_plan_bound_kind(uint32_t n, int maps);
 // Resident workgroups per CU of the batched kernel for (dk, mode, extra LDS bitmaps, lanes per workgroup).
 int batch_blocks_per_cu(DKind dk, int mode, uint32_t n, int maps, int block);
 

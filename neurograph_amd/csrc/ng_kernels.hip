@@ -363,6 +363,14 @@ __device__ __forceinline__ bool above_bound(typename T::B c, uint16_t bd) {
     return T::val(c) > static_cast<double>(__uint_as_float(static_cast<uint32_t>(bd) << 16));
 }
 
+// Where 2 bytes per node do not fit, 4-bit codes: code(d) = floor(val(d) * inv_q) capped at 15 (15: no bound),
+// with the scale inv_q = 15 / (a little more than the graph's typical largest distance). A candidate c with
+// val(c) * inv_q >= code + 1 is above the distance: the rounded product is monotone in its argument, so
+// product(c) > product(d) implies c > d, and a code stored for an earlier, larger distance only weakens the
+// test. Two nodes share a byte; a byte store that races with its neighbour's may restore the neighbour's
+// earlier code, which is still a valid (weaker) bound.
+__device__ __forceinline__ uint32_t code4(double x) { return x < 15.0 ? static_cast<uint32_t>(x) : 15u; }
+
 template <class D>
 struct KArgs {
     using B = typename DT<D>::B;
@@ -400,6 +408,8 @@ struct KArgs {
     const uint64_t* flush_base;
     uint32_t bm_words;  // words of the LDS bitmap of finalized nodes (0: no bitmap)
     uint32_t bd_words;  // words of the LDS distance bounds after the bitmaps (0: no bounds)
+    uint32_t bd_kind;   // 1: 16-bit bounds, 2: 4-bit codes (scale bd_inv_q)
+    double bd_inv_q;
     uint32_t maps;      // extra LDS bitmaps: bit 0 orders each bucket's settled nodes by id, bit 1 marks the next frontier
     double* grad1;
     double* grad2;
@@ -482,9 +492,33 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
     // phase can be pushed once more.
     const bool use_fb = use_bm && (a.maps & 2u) != 0;
     uint32_t* const fb = bm + a.bm_words * (sorted ? 2u : 1u);
-    // With `use_bd`, 16-bit distance bounds (bound16) of every node follow the bitmaps.
+    // With `use_bd`, distance bounds of every node follow the bitmaps: 16-bit (bound16) or 4-bit codes (code4).
     const bool use_bd = use_bm && a.bd_words != 0;
+    const bool bd4 = use_bd && a.bd_kind == 2;
+    const double inv_q = a.bd_inv_q;
     uint16_t* const fl = reinterpret_cast<uint16_t*>(bm + a.bm_words * (1u + (sorted ? 1u : 0u) + (use_fb ? 1u : 0u)));
+    uint8_t* const fl8 = reinterpret_cast<uint8_t*>(fl);
+    // candidate c into t is above t's distance
+    auto bd_skip = [&](uint32_t t, B c) -> bool {
+        if (!use_bd) return false;
+        if (bd4) {
+            const uint32_t code = (static_cast<uint32_t>(fl8[t >> 1]) >> ((t & 1u) * 4u)) & 15u;
+            return code < 15u && T::val(c) * inv_q >= static_cast<double>(code + 1u);
+        }
+        return above_bound<T>(c, fl[t]);
+    };
+    // t's distance was lowered to d
+    auto bd_set = [&](uint32_t t, B d) {
+        if (!use_bd) return;
+        if (bd4) {
+            const uint32_t code = code4(T::val(d) * inv_q);
+            const uint32_t sh = (t & 1u) * 4u;
+            const uint32_t old = fl8[t >> 1];
+            if (code < ((old >> sh) & 15u)) fl8[t >> 1] = static_cast<uint8_t>((old & ~(15u << sh)) | (code << sh));
+        } else {
+            fl[t] = bound16<T>(d);
+        }
+    };
     auto fb_first = [fb](uint32_t t) -> bool {
         const uint32_t bit = 1u << (t & 31);
         return (atomicOr(&fb[t >> 5], bit) & bit) == 0u;
@@ -594,8 +628,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
             for (uint32_t w = tid; w < bmw; w += BS) fb[w] = 0u;
         if (use_bd) {
             uint32_t* const flw = reinterpret_cast<uint32_t*>(fl);
-            const uint32_t bdw = (n + 1) / 2;
-            for (uint32_t w = tid; w < bdw; w += BS) flw[w] = (static_cast<uint32_t>(kBoundInf) << 16) | kBoundInf;
+            const uint32_t bdw = bd4 ? (n + 7) / 8 : (n + 1) / 2;
+            const uint32_t fillv = bd4 ? 0xFFFFFFFFu : (static_cast<uint32_t>(kBoundInf) << 16) | kBoundInf;
+            for (uint32_t w = tid; w < bdw; w += BS) flw[w] = fillv;
         }
         bsync();
         if (tid == 0) {
@@ -603,7 +638,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
             for (int k = 0; k < 8; ++k) S.cnt[k] = 0;
             S.runmin = T::kInf;
             stc(DP(s), B(0));
-            if (use_bd) fl[s] = 0;
+            bd_set(s, B(0));
             Fa[0] = s;
         }
         uint32_t* Fc = Fa;
@@ -634,7 +669,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
         // above the target's bound
         auto probe = [&](uint32_t t, L l, B dv, bool& fin, B& cur) {
             fin = use_bm && skip_ok && !(l < delta) && ((bm[t >> 5] >> (t & 31)) & 1u);
-            if (use_bd && !fin) fin = above_bound<T>(T::add(dv, l), fl[t]);
+            if (!fin) fin = bd_skip(t, T::add(dv, l));
             cur = fin ? B(0) : ldc(DP(t));
             c_probe += fin ? 0u : 1u;
         };
@@ -648,7 +683,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
                 const B old = wg_min(DP(t), nd);
                 if (nd < old) {
                     ++c_improve;
-                    if (use_bd) fl[t] = bound16<T>(nd);
+                    bd_set(t, nd);
                     if (nd < hi) {
                         pf = use_fb ? fb_first(t) : wg_exch(&fstamp[t], nph) != nph;
                     } else {
@@ -960,7 +995,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
             c_logged += nlog;
             // arc e into t is tight when its candidate c equals the final distance of t
             uint32_t n_tight = 0;
-            auto tight = [&](uint32_t t, B c) -> bool { return !(use_bd && above_bound<T>(c, fl[t])) && c == ldc(DP(t)); };
+            auto tight = [&](uint32_t t, B c) -> bool { return !bd_skip(t, c) && c == ldc(DP(t)); };
             auto mark = [&](uint32_t t, uint32_t e, B c) {
                 if (tight(t, c)) {
                     wg_or(&bits[e >> 5], 1u << (e & 31));
@@ -1218,12 +1253,14 @@ hipError_t fill(T* p, uint64_t count, T value, hipStream_t stream) {
 // MAP_FRONTIER).
 uint32_t lds_maps(int maps) { return 1u + (maps & MAP_SORTED ? 1u : 0u) + (maps & MAP_FRONTIER ? 1u : 0u); }
 
-// Dynamic LDS of the batched kernel for n nodes: the bitmaps (none when they exceed 96 KB), then the 16-bit
-// distance bounds with MAP_BOUND when everything stays within 136 KB (the static LDS takes at most ~21 KB
-// of the 160 KB of a CU).
+// Dynamic LDS of the batched kernel for n nodes: the bitmaps (none when they exceed 96 KB), then with
+// MAP_BOUND the 16-bit distance bounds, or with MAP_BOUND4 the 4-bit codes where those do not fit, as long
+// as everything stays within 136 KB (the static LDS takes at most ~21 KB of the 160 KB of a CU). The plan's
+// maps are the requested ones that fit.
 struct LdsPlan {
     uint32_t bm_words = 0;
     uint32_t bd_words = 0;
+    uint32_t bd_kind = 0;
     int maps = 0;
     size_t bytes = 0;
 };
@@ -1235,12 +1272,18 @@ LdsPlan lds_plan(uint32_t n, int maps) {
     p.bm_words = w;
     p.maps = maps & (MAP_SORTED | MAP_FRONTIER);
     p.bytes = bm_bytes;
-    const uint32_t bw = (n + 1) / 2;
-    if ((maps & MAP_BOUND) != 0 && bm_bytes + static_cast<size_t>(bw) * 4 <= 136 * 1024) {
-        p.bd_words = bw;
-        p.maps |= MAP_BOUND;
-        p.bytes += static_cast<size_t>(bw) * 4;
+    constexpr size_t kBudget = 136 * 1024;
+    const uint32_t b16 = (n + 1) / 2, b4 = (n + 7) / 8;
+    if ((maps & MAP_BOUND) != 0 && bm_bytes + static_cast<size_t>(b16) * 4 <= kBudget) {
+        p.bd_words = b16;
+        p.bd_kind = 1;
+        p.maps |= maps & (MAP_BOUND | MAP_BOUND4);
+    } else if ((maps & MAP_BOUND4) != 0 && bm_bytes + static_cast<size_t>(b4) * 4 <= kBudget) {
+        p.bd_words = b4;
+        p.bd_kind = 2;
+        p.maps |= maps & (MAP_BOUND | MAP_BOUND4);
     }
+    p.bytes += static_cast<size_t>(p.bd_words) * 4;
     return p;
 }
 
@@ -1300,6 +1343,8 @@ hipError_t launch_typed(const LaunchArgs& la, hipStream_t stream) {
     const LdsPlan lp = lds_plan(la.n, la.maps);
     k.bm_words = lp.bm_words;
     k.bd_words = lp.bd_words;
+    k.bd_kind = lp.bd_kind;
+    k.bd_inv_q = la.bd_inv_q;
     k.maps = static_cast<uint32_t>(lp.maps);
     hipLaunchKernelGGL((batch_kernel<D, MODE, BS>), dim3(la.grid), dim3(BS), lp.bytes, stream, k);
     return hipGetLastError();
@@ -1362,6 +1407,9 @@ hipError_t launch_batch(const LaunchArgs& a, hipStream_t stream) {
     }
     return hipErrorInvalidValue;
 }
+
+int lds_plan_maps(uint32_t n, int maps) { return lds_plan(n, maps).maps; }
+int lds_plan_bound_kind(uint32_t n, int maps) { return static_cast<int>(lds_plan(n, maps).bd_kind); }
 
 int batch_blocks_per_cu(DKind dk, int mode, uint32_t n, int maps, int block) {
     switch (dk) {
