@@ -204,6 +204,8 @@ struct Lds {
     uint32_t t_beg[BS];
     uint32_t t_off[BS];
     B t_dv[BS];
+    // owner (tile index) of the first arc of each chunk of 2^cs tile arcs, plus a sentinel
+    uint16_t t_own[2 * BS + 1];
     uint32_t wave[kW];
     uint32_t cnt[8];
     B runmin;
@@ -264,9 +266,10 @@ __device__ __forceinline__ void expand(const uint32_t* list, uint32_t count, Lds
 
 // Arc loop of the relaxation passes with U arcs in flight per lane: owners are located for all U arcs,
 // then all U CSR loads are issued, then all U distance probes, and only then are the results applied, so
-// each lane keeps U independent gathers outstanding instead of one. The owner search carries its lower
-// bound because a lane's arc index only grows. apply() runs on every active lane (it may append to
-// queues with wave ballots), with ok = false for padding slots.
+// each lane keeps U independent gathers outstanding instead of one. The owner of arc k lies between the
+// owners of the first arcs of its chunk and of the next chunk (a per-tile table of at most 2 BS chunks), so
+// the search is short instead of log2(BS) dependent LDS reads. apply() runs on every active lane (it may
+// append to queues with wave ballots), with ok = false for padding slots.
 #ifndef NG_UNROLL
 #define NG_UNROLL 2
 #endif
@@ -288,8 +291,18 @@ __device__ __forceinline__ void expand_relax(const uint32_t* list, uint32_t coun
         S.t_dv[tid] = dv;
         S.t_beg[tid] = beg;
         S.t_off[tid] = ex;
+        // chunks of 2^cs arcs, cs >= 6 and at most 2 BS chunks (uniform: total is the block-wide count)
+        uint32_t cs = 6;
+        while (((static_cast<uint64_t>(total) + (1ull << cs) - 1ull) >> cs) > 2u * BS) ++cs;
+        const uint32_t nq = static_cast<uint32_t>((static_cast<uint64_t>(total) + (1ull << cs) - 1ull) >> cs);
+        if (cnt != 0u) {
+            // this row owns the chunks whose first arc falls in [ex, ex + cnt)
+            const uint64_t end = static_cast<uint64_t>(ex) + cnt;
+            for (uint64_t q = (static_cast<uint64_t>(ex) + (1ull << cs) - 1ull) >> cs; (q << cs) < end; ++q)
+                S.t_own[q] = static_cast<uint16_t>(tid);
+        }
+        if (tid == 0) S.t_own[nq] = static_cast<uint16_t>(BS - 1);
         __syncthreads();
-        uint32_t lo_carry = 0;
         for (uint32_t k0 = tid; k0 < total; k0 += U * BS) {
             uint32_t pv[U], pe[U], pt[U];
             B pdv[U], pcur[U];
@@ -307,12 +320,12 @@ __device__ __forceinline__ void expand_relax(const uint32_t* list, uint32_t coun
                 pfin[j] = true;
                 pcur[j] = 0;
                 if (pok[j]) {
-                    uint32_t lo = lo_carry, hi = BS;
+                    const uint32_t q = k >> cs;
+                    uint32_t lo = S.t_own[q], hi = static_cast<uint32_t>(S.t_own[q + 1]) + 1u;
                     while (hi - lo > 1) {
                         const uint32_t mid = (lo + hi) >> 1;
                         if (S.t_off[mid] <= k) lo = mid; else hi = mid;
                     }
-                    lo_carry = lo;
                     pv[j] = S.t_v[lo];
                     pdv[j] = S.t_dv[lo];
                     pe[j] = S.t_beg[lo] + (k - S.t_off[lo]);
