@@ -272,7 +272,6 @@ void free_view(View& v) {
     if (v.len) (void)hipFree(v.len);
     v.len = nullptr;
     dfree(v.perm);
-    dfree(v.src);
     dfree(v.lend);
     v.built = false;
 }

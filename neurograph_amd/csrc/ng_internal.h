@@ -31,7 +31,6 @@ struct View {
     uint32_t* tgt = nullptr;   // [m]
     void* len = nullptr;       // [m], element type dk
     uint32_t* perm = nullptr;  // [m]
-    uint32_t* src = nullptr;   // [m] source node of each sorted arc (used by the tight-arc log fallback)
     double min_len = 0, max_len = 0, mean_len = 0, q05_len = 0;  // over finite lengths
     bool has_zero = false;
     double tuned_delta[2] = {0.0, 0.0};  // per mode (0 SSSP, 1 Brandes); 0 = not tuned
