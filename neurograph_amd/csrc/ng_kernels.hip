@@ -265,17 +265,18 @@ __device__ __forceinline__ void expand(const uint32_t* list, uint32_t count, Lds
 }
 
 // Arc loop of the relaxation passes with U arcs in flight per lane: owners are located for all U arcs,
-// then all U CSR loads are issued, then all U distance probes, and only then are the results applied, so
-// each lane keeps U independent gathers outstanding instead of one. The owner of arc k lies between the
+// then all U CSR loads are issued, then all U distance probes, then all U distance updates (returning
+// atomics), and only then are the results applied, so each lane keeps U independent accesses outstanding
+// at every step instead of one. The owner of arc k lies between the
 // owners of the first arcs of its chunk and of the next chunk (a per-tile table of at most 2 BS chunks), so
 // the search is short instead of log2(BS) dependent LDS reads. apply() runs on every active lane (it may
 // append to queues with wave ballots), with ok = false for padding slots.
 #ifndef NG_UNROLL
 #define NG_UNROLL 2
 #endif
-template <int U, int BS, class B, class L, class NodeFn, class ProbeFn, class ApplyFn>
+template <int U, int BS, class B, class L, class NodeFn, class ProbeFn, class LowerFn, class ApplyFn>
 __device__ __forceinline__ void expand_relax(const uint32_t* list, uint32_t count, Lds<B, BS>& S, const uint32_t* tgt,
-                                             const L* len, NodeFn node_fn, ProbeFn probe, ApplyFn apply) {
+                                             const L* len, NodeFn node_fn, ProbeFn probe, LowerFn lower, ApplyFn apply) {
     const uint32_t tid = threadIdx.x;
     for (uint32_t base = 0; base < count; base += BS) {
         const uint32_t i = base + tid;
@@ -342,8 +343,11 @@ __device__ __forceinline__ void expand_relax(const uint32_t* list, uint32_t coun
             for (int j = 0; j < U; ++j) {
                 if (pok[j]) probe(pt[j], pl[j], pdv[j], pfin[j], pcur[j]);
             }
+            B pold[U];
 #pragma unroll
-            for (int j = 0; j < U; ++j) apply(pok[j], pv[j], pdv[j], pe[j], pt[j], pl[j], pfin[j], pcur[j]);
+            for (int j = 0; j < U; ++j) pold[j] = lower(pok[j], pt[j], pdv[j], pl[j], pfin[j], pcur[j]);
+#pragma unroll
+            for (int j = 0; j < U; ++j) apply(pok[j], pv[j], pdv[j], pe[j], pt[j], pl[j], pfin[j], pcur[j], pold[j]);
         }
         __syncthreads();
     }
@@ -686,32 +690,36 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
             cur = fin ? B(0) : ldc(DP(t));
             c_probe += fin ? 0u : 1u;
         };
-        // relaxation of arc e = (v, t) with length l from v at distance dv, given the probe (fin, cur); new
-        // frontier entries are stamped `nph`; in the settle pass the candidate tight arcs are logged
-        auto apply = [&](bool ok, uint32_t v, B dv, uint32_t e, uint32_t t, L l, bool final_t, B cur, uint32_t nph,
+        // distance update of arc (v, t) with length l from v at distance dv, given the probe (fin, cur): t's
+        // previous distance, or the candidate itself when no update was needed
+        auto lower = [&](bool ok, uint32_t t, B dv, L l, bool final_t, B cur) -> B {
+            const B nd = T::add(dv, l);
+            return (ok && !final_t && nd < cur) ? wg_min(DP(t), nd) : nd;
+        };
+        // relaxation of arc e = (v, t) with length l from v at distance dv, given the probe (fin, cur) and the
+        // update (old); new frontier entries are stamped `nph`; in the settle pass the candidate tight arcs are
+        // logged
+        auto apply = [&](bool ok, uint32_t v, B dv, uint32_t e, uint32_t t, L l, bool final_t, B cur, B old, uint32_t nph,
                          bool log_it_if_close) {
             const B nd = T::add(dv, l);
             bool pf = false, pp = false, parked = false;
-            if (ok && !final_t && nd < cur) {
-                const B old = wg_min(DP(t), nd);
-                if (nd < old) {
-                    ++c_improve;
-                    bd_set(t, nd);
-                    if (nd < hi) {
-                        pf = use_fb ? fb_first(t) : wg_exch(&fstamp[t], nph) != nph;
+            if (nd < old) {
+                ++c_improve;
+                bd_set(t, nd);
+                if (nd < hi) {
+                    pf = use_fb ? fb_first(t) : wg_exch(&fstamp[t], nph) != nph;
+                } else {
+                    parked = true;
+                    // first parking of t in this source (its first reach: every later reach finds a finite
+                    // distance) always fits, the pile keeps n entries in reserve; a re-parking with the
+                    // improved value only while the reserve is untouched, else the next bucket advance
+                    // reloads the distances (exact mode)
+                    if (old == T::kInf) {
+                        pp = true;
+                    } else if (S.cnt[1] + BS < pcap - n) {
+                        pp = true;
                     } else {
-                        parked = true;
-                        // first parking of t in this source (its first reach: every later reach finds a finite
-                        // distance) always fits, the pile keeps n entries in reserve; a re-parking with the
-                        // improved value only while the reserve is untouched, else the next bucket advance
-                        // reloads the distances (exact mode)
-                        if (old == T::kInf) {
-                            pp = true;
-                        } else if (S.cnt[1] + BS < pcap - n) {
-                            pp = true;
-                        } else {
-                            S.cnt[5] = 1u;
-                        }
+                        S.cnt[5] = 1u;
                     }
                 }
             }
@@ -763,9 +771,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
                         cnt = lend[v] - beg;
                         c_light += cnt;
                     },
-                    probe,
-                    [&](bool ok, uint32_t v, B dv, uint32_t e, uint32_t t, L l, bool fin, B cur) {
-                        apply(ok, v, dv, e, t, l, fin, cur, nph, false);
+                    probe, lower,
+                    [&](bool ok, uint32_t v, B dv, uint32_t e, uint32_t t, L l, bool fin, B cur, B old) {
+                        apply(ok, v, dv, e, t, l, fin, cur, old, nph, false);
                     });
                 // first visit in this bucket -> settled list
                 for (uint32_t base = 0; base < nF; base += BS) {
@@ -833,9 +841,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
                         cnt = off[v + 1] - beg;
                         c_settle += cnt;
                     },
-                    probe,
-                    [&](bool ok, uint32_t v, B dv, uint32_t e, uint32_t t, L l, bool fin, B cur) {
-                        apply(ok, v, dv, e, t, l, fin, cur, nph, true);
+                    probe, lower,
+                    [&](bool ok, uint32_t v, B dv, uint32_t e, uint32_t t, L l, bool fin, B cur, B old) {
+                        apply(ok, v, dv, e, t, l, fin, cur, old, nph, true);
                     });
                 bsync();
                 NG_TOCK(t_settle, 1);
