@@ -129,6 +129,15 @@ template <class T>
 __device__ __forceinline__ T wg_and(T* p, T v) {
     return __hip_atomic_fetch_and(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
+// Graph arrays live in global memory and are read through pointers of the global address space. A pointer
+// loaded from the cohort's graph table would otherwise be generic, and generic (flat) loads also count
+// against the LDS wait counter, so every LDS wait would also wait for the loads in flight.
+template <class T>
+using gptr = const __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ gptr<T> as_global(const T* p) {
+    return (gptr<T>)p;
+}
 // Phase boundary: a workgroup barrier with workgroup-scope release/acquire.
 __device__ __forceinline__ void bsync() { __syncthreads(); }
 
@@ -275,8 +284,8 @@ __device__ __forceinline__ void expand(const uint32_t* list, uint32_t count, Lds
 #define NG_UNROLL 2
 #endif
 template <int U, int BS, class B, class L, class NodeFn, class ProbeFn, class LowerFn, class ApplyFn>
-__device__ __forceinline__ void expand_relax(const uint32_t* list, uint32_t count, Lds<B, BS>& S, const uint32_t* tgt,
-                                             const L* len, NodeFn node_fn, ProbeFn probe, LowerFn lower, ApplyFn apply) {
+__device__ __forceinline__ void expand_relax(const uint32_t* list, uint32_t count, Lds<B, BS>& S, gptr<uint32_t> tgt,
+                                             gptr<L> len, NodeFn node_fn, ProbeFn probe, LowerFn lower, ApplyFn apply) {
     const uint32_t tid = threadIdx.x;
     for (uint32_t base = 0; base < count; base += BS) {
         const uint32_t i = base + tid;
@@ -548,11 +557,11 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
     const uint32_t stride = a.stride;
     const size_t so = static_cast<size_t>(slot) * stride;
     uint32_t n = a.n;
-    const uint32_t* off = a.off;
-    const uint32_t* tgt = a.tgt;
-    const L* len = a.len;
-    const uint32_t* lend = a.lend;
-    const uint32_t* perm = a.perm;
+    gptr<uint32_t> off = as_global(a.off);
+    gptr<uint32_t> tgt = as_global(a.tgt);
+    gptr<L> len = as_global(a.len);
+    gptr<uint32_t> lend = as_global(a.lend);
+    gptr<uint32_t> perm = as_global(a.perm);
     uint32_t bmw = a.bm_words;  // bitmap words of the current graph
     B* const dist = a.dist + so;
     auto DP = [dist](uint32_t v) -> B* { return dist + v; };
@@ -613,11 +622,11 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
                 if (cur_g != 0xFFFFFFFFu) flush_bc();
                 const CohortGraph& cg = a.gtab[lo];
                 n = cg.n;
-                off = cg.off;
-                tgt = cg.tgt;
-                len = static_cast<const L*>(cg.len);
-                lend = cg.lend;
-                perm = cg.perm;
+                off = as_global(cg.off);
+                tgt = as_global(cg.tgt);
+                len = as_global(static_cast<const L*>(cg.len));
+                lend = as_global(cg.lend);
+                perm = as_global(cg.perm);
                 delta = static_cast<L>(cg.delta);
                 bmw = use_bm ? (n + 31) / 32 : 0u;
                 cur_g = lo;
