@@ -163,17 +163,22 @@ __device__ __forceinline__ T wave_reduce(T x, Op op) {
     return x;
 }
 
+// Base index of a wave-aggregated append of `n` entries: one LDS atomic by the first active lane, broadcast
+// as a scalar (no cross-lane permute).
+__device__ __forceinline__ uint32_t wave_reserve(uint32_t* lds_cnt, uint32_t n) {
+    uint32_t base = 0;
+    if (lane_id() == static_cast<uint32_t>(__ffsll(static_cast<long long>(__activemask())) - 1)) base = atomicAdd(lds_cnt, n);
+    return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(base)));
+}
+
 // Wave-aggregated append of `val` for lanes with `pred` into list[cnt++]; all active lanes must call it.
 // Appends at or beyond `cap` are dropped (the count still grows, so the caller can detect it).
 __device__ __forceinline__ void push(bool pred, uint32_t val, uint32_t* list, uint32_t* lds_cnt, uint32_t cap) {
     const unsigned long long mask = __ballot(pred);
     if (mask == 0ull) return;
     const uint32_t lane = lane_id();
-    const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<long long>(mask)) - 1);
     const uint32_t rank = static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(lds_cnt, static_cast<uint32_t>(__popcll(mask)));
-    base = __shfl(base, static_cast<int>(leader), 64);
+    const uint32_t base = wave_reserve(lds_cnt, static_cast<uint32_t>(__popcll(mask)));
     if (pred && base + rank < cap) list[base + rank] = val;
 }
 
@@ -193,11 +198,8 @@ __device__ __forceinline__ void push_entry(bool pred, B val, uint32_t node, Pile
     const unsigned long long mask = __ballot(pred);
     if (mask == 0ull) return;
     const uint32_t lane = lane_id();
-    const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<long long>(mask)) - 1);
     const uint32_t rank = static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(lds_cnt, static_cast<uint32_t>(__popcll(mask)));
-    base = __shfl(base, static_cast<int>(leader), 64);
+    const uint32_t base = wave_reserve(lds_cnt, static_cast<uint32_t>(__popcll(mask)));
     if (pred && base + rank < cap) {
         PileEntry<B> e;
         e.val = val;
@@ -744,11 +746,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
                 const unsigned long long mask = __ballot(lg);
                 if (mask != 0ull) {
                     const uint32_t lane = lane_id();
-                    const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<long long>(mask)) - 1);
                     const uint32_t rank = static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
-                    uint32_t base = 0;
-                    if (lane == leader) base = atomicAdd(&S.cnt[3], static_cast<uint32_t>(__popcll(mask)));
-                    base = __shfl(base, static_cast<int>(leader), 64);
+                    const uint32_t base = wave_reserve(&S.cnt[3], static_cast<uint32_t>(__popcll(mask)));
                     if (lg && base + rank < a.logcap) log[base + rank] = LogEntry{t, e, static_cast<unsigned long long>(nd)};
                 }
             }
