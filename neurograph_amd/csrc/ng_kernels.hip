@@ -399,6 +399,142 @@ __device__ __forceinline__ bool above_bound(typename T::B c, uint16_t bd) {
 // earlier code, which is still a valid (weaker) bound.
 __device__ __forceinline__ uint32_t code4(double x) { return x < 15.0 ? static_cast<uint32_t>(x) : 15u; }
 
+// Settle-pass form of the arc loop, for passes where few arcs survive the LDS filter (bitmap and bounds).
+// Stage A: UA arcs per lane get their owner, CSR entry and candidate, and are filtered. The survivors of the
+// wave (ranked slot by slot from ballots alone) are moved to consecutive lanes by forward permutes, one full
+// permutation per slot (non-survivors fill the other lanes), so stage B (distance probe, update, apply)
+// runs once for up to 64 survivors instead of once per slot with a few live lanes. With more survivors than
+// active lanes, stage B runs slot by slot as in expand_relax. All active lanes call apply (it uses ballots).
+template <class B>
+__device__ __forceinline__ B permute_lanes(int addr, B x) {
+    if constexpr (sizeof(B) == 8) {
+        const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_ds_permute(addr, static_cast<int>(static_cast<uint32_t>(x))));
+        const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_ds_permute(addr, static_cast<int>(static_cast<uint32_t>(x >> 32))));
+        return (static_cast<B>(hi) << 32) | lo;
+    } else {
+        return static_cast<B>(__builtin_amdgcn_ds_permute(addr, static_cast<int>(x)));
+    }
+}
+
+template <int UA, int BS, class B, class L, class NodeFn, class AddFn, class SkipFn, class ProbeFn, class LowerFn, class ApplyFn>
+__device__ __forceinline__ void expand_relax_compact(const uint32_t* list, uint32_t count, Lds<B, BS>& S, gptr<uint32_t> tgt,
+                                                     gptr<L> len, NodeFn node_fn, AddFn add, SkipFn skip, ProbeFn probe,
+                                                     LowerFn lower, ApplyFn apply) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = lane_id();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (uint32_t base = 0; base < count; base += BS) {
+        const uint32_t i = base + tid;
+        uint32_t v = 0, beg = 0, cnt = 0;
+        B dv = 0;
+        if (i < count) {
+            v = list[i];
+            node_fn(v, dv, beg, cnt);
+        }
+        uint32_t total;
+        const uint32_t ex = block_excl_scan<BS>(cnt, &total, S.wave);
+        S.t_v[tid] = v;
+        S.t_dv[tid] = dv;
+        S.t_beg[tid] = beg;
+        S.t_off[tid] = ex;
+        uint32_t cs = 6;
+        while (((static_cast<uint64_t>(total) + (1ull << cs) - 1ull) >> cs) > 2u * BS) ++cs;
+        const uint32_t nq = static_cast<uint32_t>((static_cast<uint64_t>(total) + (1ull << cs) - 1ull) >> cs);
+        if (cnt != 0u) {
+            const uint64_t end = static_cast<uint64_t>(ex) + cnt;
+            for (uint64_t q = (static_cast<uint64_t>(ex) + (1ull << cs) - 1ull) >> cs; (q << cs) < end; ++q)
+                S.t_own[q] = static_cast<uint16_t>(tid);
+        }
+        if (tid == 0) S.t_own[nq] = static_cast<uint16_t>(BS - 1);
+        __syncthreads();
+        for (uint32_t k0 = tid; k0 < total; k0 += UA * BS) {
+            uint32_t st[UA], se[UA], so[UA];
+            B snd[UA];
+            L sl[UA];
+            bool sv[UA];
+#pragma unroll
+            for (int j = 0; j < UA; ++j) {
+                const uint32_t k = k0 + static_cast<uint32_t>(j) * BS;
+                sv[j] = k < total;
+                se[j] = 0;
+                so[j] = 0;
+                if (sv[j]) {
+                    const uint32_t q = k >> cs;
+                    uint32_t lo = S.t_own[q], hi = static_cast<uint32_t>(S.t_own[q + 1]) + 1u;
+                    while (hi - lo > 1) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (S.t_off[mid] <= k) lo = mid; else hi = mid;
+                    }
+                    so[j] = lo;
+                    se[j] = S.t_beg[lo] + (k - S.t_off[lo]);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < UA; ++j) {
+                st[j] = 0;
+                sl[j] = L(0);
+                if (sv[j]) {
+                    st[j] = tgt[se[j]];
+                    sl[j] = len[se[j]];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < UA; ++j) {
+                snd[j] = 0;
+                if (sv[j]) {
+                    snd[j] = add(S.t_dv[so[j]], sl[j]);
+                    sv[j] = !skip(st[j], sl[j], snd[j]);
+                }
+            }
+            unsigned long long m[UA];
+            uint32_t o[UA + 1];
+            o[0] = 0;
+#pragma unroll
+            for (int j = 0; j < UA; ++j) {
+                m[j] = __ballot(sv[j]);
+                o[j + 1] = o[j] + static_cast<uint32_t>(__popcll(m[j]));
+            }
+            const uint32_t nsv = o[UA];
+            if (nsv == 0) continue;
+            // compaction needs a receiving lane per survivor: at most 64, all of them active
+            const unsigned long long need = nsv >= 64 ? ~0ull : ((1ull << nsv) - 1ull);
+            if (nsv <= 64 && (__activemask() & need) == need) {
+                // survivors to lanes [0, nsv): slot j's to [o[j], o[j+1]), the rest of the lanes to the others
+                uint32_t ct = 0, ce = 0;
+                B cnd = 0;
+#pragma unroll
+                for (int j = 0; j < UA; ++j) {
+                    const uint32_t c = o[j + 1] - o[j];
+                    const uint32_t dst = sv[j] ? o[j] + static_cast<uint32_t>(__popcll(m[j] & below))
+                                               : (o[j] + c + static_cast<uint32_t>(__popcll(~m[j] & below))) & 63u;
+                    const int addr = static_cast<int>(dst << 2);
+                    const uint32_t rt = static_cast<uint32_t>(__builtin_amdgcn_ds_permute(addr, static_cast<int>(st[j])));
+                    const uint32_t re = static_cast<uint32_t>(__builtin_amdgcn_ds_permute(addr, static_cast<int>(se[j])));
+                    const B rn = permute_lanes<B>(addr, snd[j]);
+                    if (lane >= o[j] && lane < o[j + 1]) {
+                        ct = rt;
+                        ce = re;
+                        cnd = rn;
+                    }
+                }
+                const bool live = lane < nsv;
+                const B cur = live ? probe(ct) : B(0);
+                const B old = lower(live, ct, cnd, cur);
+                apply(live, ce, ct, cnd, cur, old);
+            } else {
+                B cur[UA], old[UA];
+#pragma unroll
+                for (int j = 0; j < UA; ++j) cur[j] = sv[j] ? probe(st[j]) : B(0);
+#pragma unroll
+                for (int j = 0; j < UA; ++j) old[j] = lower(sv[j], st[j], snd[j], cur[j]);
+#pragma unroll
+                for (int j = 0; j < UA; ++j) apply(sv[j], se[j], st[j], snd[j], cur[j], old[j]);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 template <class D>
 struct KArgs {
     using B = typename DT<D>::B;
@@ -710,9 +846,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
         // relaxation of arc e = (v, t) with length l from v at distance dv, given the probe (fin, cur) and the
         // update (old); new frontier entries are stamped `nph`; in the settle pass the candidate tight arcs are
         // logged
-        auto apply = [&](bool ok, uint32_t v, B dv, uint32_t e, uint32_t t, L l, bool final_t, B cur, B old, uint32_t nph,
-                         bool log_it_if_close) {
-            const B nd = T::add(dv, l);
+        // (live: the arc was probed; nd: its candidate distance)
+        auto apply_core = [&](bool live, uint32_t e, uint32_t t, B nd, B cur, B old, uint32_t nph, bool log_it_if_close) {
             bool pf = false, pp = false, parked = false;
             if (nd < old) {
                 ++c_improve;
@@ -742,7 +877,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
                 if (lane_id() == 0) atomicMin(&S.runmin, m);
             }
             if constexpr (MODE != MODE_SSSP) {
-                const bool lg = ok && log_it_if_close && !final_t && nd <= cur;
+                const bool lg = live && log_it_if_close && nd <= cur;
                 const unsigned long long mask = __ballot(lg);
                 if (mask != 0ull) {
                     const uint32_t lane = lane_id();
@@ -752,6 +887,21 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
                 }
             }
         };
+        auto apply = [&](bool ok, uint32_t, B dv, uint32_t e, uint32_t t, L l, bool final_t, B cur, B old, uint32_t nph,
+                         bool log_it_if_close) {
+            apply_core(ok && !final_t, e, t, T::add(dv, l), cur, old, nph, log_it_if_close);
+        };
+        // the same steps for the compacted settle pass, given the candidate: the filter (heavy arc into a
+        // registered node, or candidate above the target's bound), the probe and the update
+        auto skip_arc = [&](uint32_t t, L l, B nd) -> bool {
+            if (use_bm && skip_ok && !(l < delta) && ((bm[t >> 5] >> (t & 31)) & 1u)) return true;
+            return bd_skip(t, nd);
+        };
+        auto probe_t = [&](uint32_t t) -> B {
+            ++c_probe;
+            return ldc(DP(t));
+        };
+        auto lower_nd = [&](bool live, uint32_t t, B nd, B cur) -> B { return (live && nd < cur) ? wg_min(DP(t), nd) : nd; };
 
         while (true) {
             // ---- inner phases: light arcs of the frontier until the bucket [lo, hi) is settled
@@ -841,18 +991,28 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
             if (nS > settle_from) {
                 NG_TICK(t_settle);
                 const uint32_t nph = phase + 1;
-                expand_relax<NG_UNROLL, BS>(
-                    Sl + settle_from, nS - settle_from, S, tgt, len,
-                    [&](uint32_t v, B& dv, uint32_t& beg, uint32_t& cnt) {
-                        dv = ldc(DP(v));
-                        beg = (MODE != MODE_SSSP) ? off[v] : lend[v];
-                        cnt = off[v + 1] - beg;
-                        c_settle += cnt;
-                    },
-                    probe, lower,
-                    [&](bool ok, uint32_t v, B dv, uint32_t e, uint32_t t, L l, bool fin, B cur, B old) {
-                        apply(ok, v, dv, e, t, l, fin, cur, old, nph, true);
-                    });
+                auto settle_row = [&](uint32_t v, B& dv, uint32_t& beg, uint32_t& cnt) {
+                    dv = ldc(DP(v));
+                    beg = (MODE != MODE_SSSP) ? off[v] : lend[v];
+                    cnt = off[v + 1] - beg;
+                    c_settle += cnt;
+                };
+                if constexpr (BS >= 512) {
+                    // wide workgroups: compacted survivors (the 256-lane kernel's register budget is too small)
+                    // arcs per lane in the filter stage: fewer for 8-byte distances (registers; measured 3 vs 4:
+                    // +1 % on spatial50k), more for 4-byte ones (6 vs 4: +0.6 % on spatial200k)
+                    constexpr int kFilterArcs = sizeof(B) == 8 ? 3 : 6;
+                    expand_relax_compact<kFilterArcs, BS>(
+                        Sl + settle_from, nS - settle_from, S, tgt, len, settle_row,
+                        [](B dv, L l) -> B { return T::add(dv, l); }, skip_arc, probe_t, lower_nd,
+                        [&](bool live, uint32_t e, uint32_t t, B nd, B cur, B old) { apply_core(live, e, t, nd, cur, old, nph, true); });
+                } else {
+                    expand_relax<NG_UNROLL, BS>(
+                        Sl + settle_from, nS - settle_from, S, tgt, len, settle_row, probe, lower,
+                        [&](bool ok, uint32_t v, B dv, uint32_t e, uint32_t t, L l, bool fin, B cur, B old) {
+                            apply(ok, v, dv, e, t, l, fin, cur, old, nph, true);
+                        });
+                }
                 bsync();
                 NG_TOCK(t_settle, 1);
                 settle_from = nS;
