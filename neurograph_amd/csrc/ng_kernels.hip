@@ -921,18 +921,24 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
                     }
                     bsync();
                 }
-                expand_relax<NG_UNROLL, BS>(
-                    Fc, nF, S, tgt, len,
-                    [&](uint32_t v, B& dv, uint32_t& beg, uint32_t& cnt) {
-                        dv = ldc(DP(v));
-                        beg = off[v];
-                        cnt = lend[v] - beg;
-                        c_light += cnt;
-                    },
-                    probe, lower,
-                    [&](bool ok, uint32_t v, B dv, uint32_t e, uint32_t t, L l, bool fin, B cur, B old) {
-                        apply(ok, v, dv, e, t, l, fin, cur, old, nph, false);
-                    });
+                auto light_row = [&](uint32_t v, B& dv, uint32_t& beg, uint32_t& cnt) {
+                    dv = ldc(DP(v));
+                    beg = off[v];
+                    cnt = lend[v] - beg;
+                    c_light += cnt;
+                };
+                if constexpr (BS >= 512) {
+                    expand_relax_compact<3, BS>(
+                        Fc, nF, S, tgt, len, light_row, [](B dv, L l) -> B { return T::add(dv, l); }, skip_arc, probe_t,
+                        lower_nd,
+                        [&](bool live, uint32_t e, uint32_t t, B nd, B cur, B old) { apply_core(live, e, t, nd, cur, old, nph, false); });
+                } else {
+                    expand_relax<NG_UNROLL, BS>(
+                        Fc, nF, S, tgt, len, light_row, probe, lower,
+                        [&](bool ok, uint32_t v, B dv, uint32_t e, uint32_t t, L l, bool fin, B cur, B old) {
+                            apply(ok, v, dv, e, t, l, fin, cur, old, nph, false);
+                        });
+                }
                 // first visit in this bucket -> settled list
                 for (uint32_t base = 0; base < nF; base += BS) {
                     const uint32_t i = base + tid;
