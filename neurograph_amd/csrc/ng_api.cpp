@@ -313,8 +313,12 @@ int lds_maps_for(DKind dk, int mode, uint32_t n, int block, bool allow4) {
     const int b = allow4 ? (MAP_BOUND | MAP_BOUND4) : static_cast<int>(MAP_BOUND);
     if (const char* e = std::getenv("NG_LDS_MAPS")) return std::atoi(e) & (allow4 ? 15 : 7);
     const int base = batch_blocks_per_cu(dk, mode, n, 0, block);
-    for (int maps : {b | MAP_SORTED | MAP_FRONTIER, b | MAP_FRONTIER, b | MAP_SORTED, b, MAP_SORTED | MAP_FRONTIER,
-                     static_cast<int>(MAP_FRONTIER), static_cast<int>(MAP_SORTED)})
+    // 16-bit bounds where they fit, else 4-bit codes (also where 16-bit bounds would cost workgroups: on a
+    // 50k-node graph at 512 lanes, 4-bit codes keep two workgroups per CU, 12 % faster than no bounds)
+    const int b4 = allow4 ? static_cast<int>(MAP_BOUND4) : 0;
+    for (int maps : {b | MAP_SORTED | MAP_FRONTIER, b | MAP_FRONTIER, b | MAP_SORTED, b, b4 | MAP_SORTED | MAP_FRONTIER,
+                     b4 | MAP_FRONTIER, b4 | MAP_SORTED, b4, MAP_SORTED | MAP_FRONTIER, static_cast<int>(MAP_FRONTIER),
+                     static_cast<int>(MAP_SORTED)})
         if (lds_plan_maps(n, maps) == maps && batch_blocks_per_cu(dk, mode, n, maps, block) >= base) return maps;
     return 0;
 }
