@@ -73,7 +73,9 @@ def test_repeated_sources_accumulate(ctx):
     assert s[0] == s[1] and s[1] == s[2] and s[2] == s[5]
 
 
-def test_long_directed_path(ctx):
+@pytest.mark.parametrize("block", ["256", "1024"])
+def test_long_directed_path(ctx, monkeypatch, block):
+    monkeypatch.setenv("NG_BLOCK_SIZE", block)
     n = 20000
     c = orc.Csr.from_edges(n, np.arange(n - 1), np.arange(1, n), np.ones(n - 1, np.int32))
     g = dev(ctx, c)
@@ -89,7 +91,10 @@ def test_long_directed_path(ctx):
     np.testing.assert_array_equal(bc, expect)
 
 
-def test_hub_degree_skew(ctx):
+@pytest.mark.parametrize("block", ["256", "1024"])
+def test_hub_degree_skew(ctx, monkeypatch, block):
+    """A 60 000-arc hub row: expansion tiles whose arc count outgrows the chunk owner table's 64-arc chunks."""
+    monkeypatch.setenv("NG_BLOCK_SIZE", block)
     n = 60001
     leaves = np.arange(1, n)
     rng = np.random.default_rng(8)

@@ -249,13 +249,16 @@ def test_small_pile_forces_exact_bucket_advance(ctx, monkeypatch):
         np.testing.assert_allclose(bc, orc.brandes(c, sources=srcs)["bc"], rtol=1e-9, atol=1e-9)
 
 
+@pytest.mark.parametrize("block", ["256", "1024"])
 @pytest.mark.parametrize("maps", [str(k) for k in range(16)])
-def test_lds_bitmap_variants(ctx, monkeypatch, maps):
+def test_lds_bitmap_variants(ctx, monkeypatch, maps, block):
     """Every combination of the optional LDS maps (settle passes in discovery order or in increasing id;
     frontier de-duplication by global stamps or by an LDS bitmap; probes with no bounds, 16-bit bounds or, for
-    maps 8-11, the 4-bit distance codes) gives the oracle's distances bit-exactly and its betweenness within
-    1e-9, for integer, FP32 and FP64 weights and both length kinds (|w| and 1/|w|)."""
+    maps 8-15, the 4-bit distance codes) gives the oracle's distances bit-exactly and its betweenness within
+    1e-9, for integer, FP32 and FP64 weights and both length kinds (|w| and 1/|w|), in 256-lane workgroups
+    (plain arc loop) and 1024-lane ones (filter stage, survivors compacted by lane permutes)."""
     monkeypatch.setenv("NG_LDS_MAPS", maps)
+    monkeypatch.setenv("NG_BLOCK_SIZE", block)
     # int32 weights up to 2^31 - 1 take the 64-bit integer distances, whose bounds round
     for wdtype, seed, wlo, whi in ((np.int32, 5, 1, 9), (np.int32, 9, 1 << 20, 2**31 - 1), (np.float32, 7, 1, 9),
                                    (np.float64, 6, 1, 9)):
