@@ -612,6 +612,23 @@ __device__ __forceinline__ uint32_t range_mask(uint32_t wi, uint32_t b, uint32_t
     return m;
 }
 
+// Visits the nonzero words of the tight-arc bitmap covering arcs [b, e_end), in increasing order, with f(wi, w)
+// (w masked to the row). Up to four words are loaded before any is visited: they are independent loads, and
+// a row of ~100 arcs spans four words.
+template <class F>
+__device__ __forceinline__ void for_row_words(const uint32_t* bits, uint32_t b, uint32_t e_end, F f) {
+    if (b >= e_end) return;
+    const uint32_t w1 = (e_end - 1) >> 5;
+    for (uint32_t wb = b >> 5; wb <= w1; wb += 4) {
+        uint32_t ws[4];
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) ws[q] = wb + q <= w1 ? ldc(&bits[wb + q]) & range_mask(wb + q, b, e_end) : 0u;
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q)
+            if (ws[q] != 0u) f(wb + q, ws[q]);
+    }
+}
+
 // Optional per-section wall-clock accounting (compile with -DNG_SECTION_TIMERS): workgroup-uniform
 // timestamps, summed by lane 0 into arcs_reached[8 + section]. Off in the normal build.
 #ifdef NG_SECTION_TIMERS
@@ -1253,28 +1270,24 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
                 for (uint32_t i = lb + tid; i < le; i += BS) {
                     const uint32_t u = order[i];
                     const double su = tree ? 1.0 : ldc(&R[u].sigma);
-                    const uint32_t b = off[u], e_end = off[u + 1];
-                    if (b < e_end) {
-                        for (uint32_t wi = b >> 5; wi <= (e_end - 1) >> 5; ++wi) {
-                            uint32_t w = ldc(&bits[wi]) & range_mask(wi, b, e_end);
-                            if (push_dep && w != 0u) wg_and(&bits[wi], ~w);
-                            while (w) {
-                                const uint32_t e = (wi << 5) + static_cast<uint32_t>(__ffs(w) - 1);
-                                w &= w - 1;
-                                const uint32_t t = tgt[e];
-                                if (push_dep) stc(&R[t].spare, u);
-                                bool released = true;  // a tree node's only tight in-arc releases it
-                                if (!tree) {
-                                    wg_add(&R[t].sigma, su);
-                                    released = wg_sub(&R[t].indeg, 1u) == 1u;
-                                }
-                                if (released) {
-                                    const uint32_t pos = atomicAdd(&S.cnt[4], 1u);
-                                    if (pos < n) order[pos] = t;
-                                }
+                    for_row_words(bits, off[u], off[u + 1], [&](uint32_t wi, uint32_t w) {
+                        if (push_dep) wg_and(&bits[wi], ~w);
+                        while (w) {
+                            const uint32_t e = (wi << 5) + static_cast<uint32_t>(__ffs(w) - 1);
+                            w &= w - 1;
+                            const uint32_t t = tgt[e];
+                            if (push_dep) stc(&R[t].spare, u);
+                            bool released = true;  // a tree node's only tight in-arc releases it
+                            if (!tree) {
+                                wg_add(&R[t].sigma, su);
+                                released = wg_sub(&R[t].indeg, 1u) == 1u;
+                            }
+                            if (released) {
+                                const uint32_t pos = atomicAdd(&S.cnt[4], 1u);
+                                if (pos < n) order[pos] = t;
                             }
                         }
-                    }
+                    });
                 }
                 bsync();
                 lb = le;
@@ -1315,29 +1328,24 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
                         const uint32_t u = order[i];
                         const double su = tree ? 1.0 : ldc(&R[u].sigma);
                         double acc1 = 0.0, acc2 = 0.0;
-                        const uint32_t b = off[u], e_end = off[u + 1];
-                        if (b < e_end) {
-                            for (uint32_t wi = b >> 5; wi <= (e_end - 1) >> 5; ++wi) {
-                                uint32_t w = ldc(&bits[wi]) & range_mask(wi, b, e_end);
-                                if (w == 0u) continue;
-                                wg_and(&bits[wi], ~w);
-                                while (w) {
-                                    const uint32_t e = (wi << 5) + static_cast<uint32_t>(__ffs(w) - 1);
-                                    w &= w - 1;
-                                    const uint32_t t = tgt[e];
-                                    const double st = tree ? 1.0 : ldc(&R[t].sigma);
-                                    const double term1 = su * ((1.0 + ldc(&R[t].delta1)) / st);
-                                    acc1 += term1;
-                                    if constexpr ((MODE & MODE_ARC1) != 0) unsafeAtomicAdd(&a.grad1[perm[e]], term1);
-                                    if constexpr ((MODE & MODE_ARC2) != 0) {
-                                        const double dt = T::val(ldc(DP(t)));
-                                        const double term2 = su * ((1.0 / (dt * dt) + ldc(&delta2[t])) / st);
-                                        acc2 += term2;
-                                        unsafeAtomicAdd(&a.grad2[perm[e]], term2);
-                                    }
+                        for_row_words(bits, off[u], off[u + 1], [&](uint32_t wi, uint32_t w) {
+                            wg_and(&bits[wi], ~w);
+                            while (w) {
+                                const uint32_t e = (wi << 5) + static_cast<uint32_t>(__ffs(w) - 1);
+                                w &= w - 1;
+                                const uint32_t t = tgt[e];
+                                const double st = tree ? 1.0 : ldc(&R[t].sigma);
+                                const double term1 = su * ((1.0 + ldc(&R[t].delta1)) / st);
+                                acc1 += term1;
+                                if constexpr ((MODE & MODE_ARC1) != 0) unsafeAtomicAdd(&a.grad1[perm[e]], term1);
+                                if constexpr ((MODE & MODE_ARC2) != 0) {
+                                    const double dt = T::val(ldc(DP(t)));
+                                    const double term2 = su * ((1.0 / (dt * dt) + ldc(&delta2[t])) / st);
+                                    acc2 += term2;
+                                    unsafeAtomicAdd(&a.grad2[perm[e]], term2);
                                 }
                             }
-                        }
+                        });
                         if (ly > 0) {
                             stc(&R[u].delta1, acc1);
                             if constexpr ((MODE & MODE_ARC2) != 0) stc(&delta2[u], acc2);
