@@ -1,6 +1,7 @@
 """Edge cases of the GPU path against the oracle and closed forms: empty and one-node graphs, isolated nodes,
 self-loops and parallel arcs, repeated sources, a 20 000-node directed path (one bucket per node), extreme
-degree skew (a 60 000-arc hub), integer weights that need 64-bit distances, and rejected inputs."""
+degree skew (a 60 000-arc hub), a graph too large for the LDS maps, integer weights that need 64-bit distances,
+and rejected inputs."""
 
 import numpy as np
 import pytest
@@ -103,6 +104,20 @@ def test_hub_degree_skew(ctx, monkeypatch, block):
     w = rng.integers(1, 20, len(src)).astype(np.int32)
     c = orc.Csr.from_edges(n, src, dst, w)
     check_all(ctx, c, [0, 1, 17, 60000])
+
+
+def test_graph_beyond_lds_maps(ctx, monkeypatch):
+    """800 000 nodes: no LDS bitmap or bound fits, so the settled and frontier stamps live in global memory and
+    the wide workgroups' filter stage passes every arc to the slot-by-slot form."""
+    monkeypatch.setenv("NG_BLOCK_SIZE", "1024")
+    n, m = 800_000, 2_400_000
+    rng = np.random.default_rng(12)
+    src = rng.integers(0, n, m)
+    dst = (src + rng.integers(1, 2000, m)) % n  # local arcs: long shortest paths, many buckets
+    w = rng.integers(1, 50, m).astype(np.int32)
+    c = orc.Csr.from_edges(n, src, dst, w)
+    g, _ = check_all(ctx, c, [0, 123_457, 799_999])
+    assert g.last_stats()["block"] == 1024
 
 
 def test_large_integer_weights_use_64_bit_distances(ctx):
