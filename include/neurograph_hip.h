@@ -158,7 +158,8 @@ ng_status ng_autotune_delta(ng_graph* g, ng_length_kind lkind, int mode, const u
                             double* delta_out);
 
 /* Picks the bucket width and the lanes per workgroup (256, 512 or 1024) together for (lkind, mode) by timing
- * the batched kernel on `sample` sources, and stores both in the graph (used whenever delta <= 0). Fewer,
+ * the batched kernel on `sample` sources (widths: the 5 % length quantile times 1/4 ... 8 in powers of two,
+ * then the best one times 1/sqrt(2) and sqrt(2)), and stores both in the graph (used whenever delta <= 0). Fewer,
  * wider workgroups keep fewer sources in flight, so their per-source state stays cache-resident on large
  * graphs. The sample should hold at least twice as many sources as the GPU runs at once with 256-lane
  * workgroups (5 per CU), so that every candidate is timed at full occupancy. Extension, like

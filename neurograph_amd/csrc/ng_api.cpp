@@ -984,6 +984,22 @@ ng_status ng_autotune(ng_graph* g, ng_length_kind lkind, int mode, const uint32_
             }
         }
     }
+    // refinement between the powers of two around the best width, at the best workgroup size
+    const double grid_best = best_d;
+    for (double f : {0.7071067811865476, 1.4142135623730951}) {
+        double d = grid_best * f;
+        if (integer) d = std::max(1.0, std::round(d));
+        if (std::find(cand.begin(), cand.end(), d) != cand.end()) continue;
+        cand.push_back(d);
+        RunSpec rs{vi, run_mode, d, sample, n_sample, nullptr};
+        rs.block = best_b;
+        st = run_batch(g, rs, nullptr, nullptr);
+        if (st != NG_OK) return st;
+        if (g->last.kernel_ms < best_ms) {
+            best_ms = g->last.kernel_ms;
+            best_d = d;
+        }
+    }
     v.tuned_delta[mi] = best_d;
     v.tuned_block[mi] = best_b;
     if (delta_out) *delta_out = best_d;
