@@ -745,8 +745,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
     uint32_t phase = ctr[0], gen = ctr[1], pgen = ctr[2];
     L delta = a.delta;
     // work counters (uniform per workgroup, except the arc counts which lane 0 accumulates from the tiles)
-    unsigned long long c_phases = 0, c_buckets = 0, c_light = 0, c_settle = 0, c_logged = 0, c_layers = 0;
-    unsigned long long c_probe = 0, c_improve = 0;
+    unsigned long long c_phases = 0, c_buckets = 0, c_logged = 0, c_layers = 0;
+    // per-lane counts of one launch (32 bits: a lane handles far fewer than 2^32 arcs per launch)
+    uint32_t c_light = 0, c_settle = 0, c_probe = 0, c_improve = 0;
 
     // Sources blockIdx.x, +gridDim.x, ... of a.sources (plain launch), or of the cohort's (graph, source) items
     // in graph order (cohort launch, a.gtab != nullptr): all slots sweep the member graphs together, so the
@@ -1393,15 +1394,15 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
     }
     // per-lane arc counts (each lane counted the rows it loaded)
     auto add64 = [](unsigned long long x, unsigned long long y) { return x + y; };
-    c_light = wave_reduce(c_light, add64);
-    c_settle = wave_reduce(c_settle, add64);
-    c_probe = wave_reduce(c_probe, add64);
-    c_improve = wave_reduce(c_improve, add64);
+    const unsigned long long w_light = wave_reduce(static_cast<unsigned long long>(c_light), add64);
+    const unsigned long long w_settle = wave_reduce(static_cast<unsigned long long>(c_settle), add64);
+    const unsigned long long w_probe = wave_reduce(static_cast<unsigned long long>(c_probe), add64);
+    const unsigned long long w_improve = wave_reduce(static_cast<unsigned long long>(c_improve), add64);
     if (lane_id() == 0) {
-        atomicAdd(&a.arcs_reached[3], c_light);
-        atomicAdd(&a.arcs_reached[4], c_settle);
-        atomicAdd(&a.arcs_reached[7], c_probe);
-        atomicAdd(&a.arcs_reached[8], c_improve);
+        atomicAdd(&a.arcs_reached[3], w_light);
+        atomicAdd(&a.arcs_reached[4], w_settle);
+        atomicAdd(&a.arcs_reached[7], w_probe);
+        atomicAdd(&a.arcs_reached[8], w_improve);
     }
 }
 
