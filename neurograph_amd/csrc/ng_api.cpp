@@ -311,15 +311,19 @@ void free_ws(Workspace& w) {
 // replaces the global frontier stamps. NG_LDS_MAPS=0..15 overrides the choice (tuning experiments).
 int lds_maps_for(DKind dk, int mode, uint32_t n, int block, bool allow4) {
     const int b = allow4 ? (MAP_BOUND | MAP_BOUND4) : static_cast<int>(MAP_BOUND);
-    if (const char* e = std::getenv("NG_LDS_MAPS")) return std::atoi(e) & (allow4 ? 15 : 7);
+    if (const char* e = std::getenv("NG_LDS_MAPS")) return std::atoi(e) & (allow4 ? 31 : 23);
     const int base = batch_blocks_per_cu(dk, mode, n, 0, block);
+    // small graphs in 256-lane workgroups: the distances themselves in LDS
+    if (block == 256)
+        for (int maps : {MAP_LDIST | MAP_SORTED | MAP_FRONTIER, MAP_LDIST | MAP_FRONTIER, MAP_LDIST | MAP_SORTED, static_cast<int>(MAP_LDIST)})
+            if (lds_plan_maps(dk, n, maps) == maps && batch_blocks_per_cu(dk, mode, n, maps, block) >= base) return maps;
     // 16-bit bounds where they fit, else 4-bit codes (also where 16-bit bounds would cost workgroups: on a
     // 50k-node graph at 512 lanes, 4-bit codes keep two workgroups per CU, 12 % faster than no bounds)
     const int b4 = allow4 ? static_cast<int>(MAP_BOUND4) : 0;
     for (int maps : {b | MAP_SORTED | MAP_FRONTIER, b | MAP_FRONTIER, b | MAP_SORTED, b, b4 | MAP_SORTED | MAP_FRONTIER,
                      b4 | MAP_FRONTIER, b4 | MAP_SORTED, b4, MAP_SORTED | MAP_FRONTIER, static_cast<int>(MAP_FRONTIER),
                      static_cast<int>(MAP_SORTED)})
-        if (lds_plan_maps(n, maps) == maps && batch_blocks_per_cu(dk, mode, n, maps, block) >= base) return maps;
+        if (lds_plan_maps(dk, n, maps) == maps && batch_blocks_per_cu(dk, mode, n, maps, block) >= base) return maps;
     return 0;
 }
 
@@ -484,7 +488,7 @@ ng_status run_batch(ng_graph* g, const RunSpec& rs, uint32_t* slots_used, double
     // slots: resident workgroups of this kernel, bounded by the sources and by device memory
     const int block = pick_block(v, mode_idx, rs.block);
     const int maps = lds_maps_for(v.dk, rs.mode, g->n, block, !rs.no_bound4);
-    if (lds_plan_bound_kind(g->n, maps) == 2 && !(v.bound_inv_q > 0.0)) {
+    if (lds_plan_bound_kind(v.dk, g->n, maps) == 2 && !(v.bound_inv_q > 0.0)) {
         st = estimate_bound_scale(g, rs.vi, delta);
         if (st != NG_OK) return st;
     }

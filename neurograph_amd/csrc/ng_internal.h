@@ -166,11 +166,12 @@ hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s
 // Extra LDS maps beyond the finalized-node bitmap (bit flags of LaunchArgs::maps): bitmaps of each bucket's
 // settled nodes and of the next frontier, 16-bit distance bounds per node, and 4-bit distance codes where the
 // 16-bit bounds do not fit (MAP_BOUND4 needs LaunchArgs::bd_inv_q).
-enum : int { MAP_SORTED = 1, MAP_FRONTIER = 2, MAP_BOUND = 4, MAP_BOUND4 = 8 };
-// The maps of `maps` that fit next to the kernel's static LDS for n nodes, and the bound kind they give
-// (0: none, 1: 16-bit, 2: 4-bit).
-int lds_plan_maps(uint32_t n, int maps);
-int lds_plan_bound_kind(uint32_t n, int maps);
+// MAP_LDIST keeps the current source's distances themselves in LDS (256-lane workgroups; replaces the bounds).
+enum : int { MAP_SORTED = 1, MAP_FRONTIER = 2, MAP_BOUND = 4, MAP_BOUND4 = 8, MAP_LDIST = 16 };
+// The maps of `maps` that fit next to the kernel's static LDS for n nodes of distance kind dk, and the bound
+// kind they give (0: none, 1: 16-bit, 2: 4-bit).
+int lds_plan_maps(DKind dk, uint32_t n, int maps);
+int lds_plan_bound_kind(DKind dk, uint32_t n, int maps);
 // Resident workgroups per CU of the batched kernel for (dk, mode, extra LDS bitmaps, lanes per workgroup).
 int batch_blocks_per_cu(DKind dk, int mode, uint32_t n, int maps, int block);
 

@@ -573,6 +573,7 @@ struct KArgs {
     uint32_t bm_words;  // words of the LDS bitmap of finalized nodes (0: no bitmap)
     uint32_t bd_words;  // words of the LDS distance bounds after the bitmaps (0: no bounds)
     uint32_t bd_kind;   // 1: 16-bit bounds, 2: 4-bit codes (scale bd_inv_q)
+    uint32_t ld_off;    // word offset of the distances in dynamic LDS (kernels with LDS distances)
     double bd_inv_q;
     uint32_t maps;      // extra LDS bitmaps: bit 0 orders each bucket's settled nodes by id, bit 1 marks the next frontier
     double* grad1;
@@ -648,7 +649,9 @@ __device__ __forceinline__ void for_row_words(const uint32_t* bits, uint32_t b, 
 #ifndef NG_WAVES_PER_EU
 #define NG_WAVES_PER_EU 5
 #endif
-template <class D, int MODE, int BS>
+// LD: the current source's distances live in LDS (graphs small enough; 256-lane workgroups), so probes and
+// updates are LDS operations; otherwise they live in the slot's global array.
+template <class D, int MODE, int BS, bool LD>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? NG_WAVES_PER_EU : 4))) void batch_kernel(
     const KArgs<D> a) {
     using T = DT<D>;
@@ -718,7 +721,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
     gptr<uint32_t> lend = as_global(a.lend);
     gptr<uint32_t> perm = as_global(a.perm);
     uint32_t bmw = a.bm_words;  // bitmap words of the current graph
-    B* const dist = a.dist + so;
+    B* const dist = LD ? reinterpret_cast<B*>(bm + a.ld_off) : a.dist + so;
     auto DP = [dist](uint32_t v) -> B* { return dist + v; };
     uint32_t* const fstamp = a.fstamp + so;
     uint32_t* sstamp = a.sstamp + so;
@@ -808,6 +811,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
             for (uint32_t w = tid; w < bmw; w += BS) bn[w] = 0u;
         if (use_fb)
             for (uint32_t w = tid; w < bmw; w += BS) fb[w] = 0u;
+        if constexpr (LD) {
+            for (uint32_t v = tid; v < n; v += BS) dist[v] = T::kInf;
+        }
         if (use_bd) {
             uint32_t* const flw = reinterpret_cast<uint32_t*>(fl);
             const uint32_t bdw = bd4 ? (n + 7) / 8 : (n + 1) / 2;
@@ -1465,10 +1471,13 @@ struct LdsPlan {
     uint32_t bm_words = 0;
     uint32_t bd_words = 0;
     uint32_t bd_kind = 0;
+    uint32_t ld_off = 0;
     int maps = 0;
     size_t bytes = 0;
 };
-LdsPlan lds_plan(uint32_t n, int maps) {
+// dsize: bytes per distance. With MAP_LDIST, the distances (n x dsize, 8-byte aligned) follow the bitmaps
+// instead of the bounds.
+LdsPlan lds_plan(uint32_t n, int maps, uint32_t dsize) {
     LdsPlan p;
     const uint32_t w = (n + 31) / 32;
     const size_t bm_bytes = static_cast<size_t>(w) * 4 * lds_maps(maps);
@@ -1477,6 +1486,16 @@ LdsPlan lds_plan(uint32_t n, int maps) {
     p.maps = maps & (MAP_SORTED | MAP_FRONTIER);
     p.bytes = bm_bytes;
     constexpr size_t kBudget = 136 * 1024;
+    if ((maps & MAP_LDIST) != 0) {
+        const uint32_t off = (w * lds_maps(maps) + 1u) & ~1u;
+        const size_t bytes = static_cast<size_t>(off) * 4 + static_cast<size_t>(n) * dsize;
+        if (bytes <= kBudget) {
+            p.ld_off = off;
+            p.maps |= MAP_LDIST;
+            p.bytes = bytes;
+        }
+        return p;
+    }
     const uint32_t b16 = (n + 1) / 2, b4 = (n + 7) / 8;
     if ((maps & MAP_BOUND) != 0 && bm_bytes + static_cast<size_t>(b16) * 4 <= kBudget) {
         p.bd_words = b16;
@@ -1491,7 +1510,7 @@ LdsPlan lds_plan(uint32_t n, int maps) {
     return p;
 }
 
-template <class D, int MODE, int BS>
+template <class D, int MODE, int BS, bool LD>
 hipError_t launch_typed(const LaunchArgs& la, hipStream_t stream) {
     using T = DT<D>;
     using B = typename T::B;
@@ -1544,47 +1563,50 @@ hipError_t launch_typed(const LaunchArgs& la, hipStream_t stream) {
     k.err = ws.err;
     k.dist_out = la.dist_out;
     k.dist_out_kind = la.dist_out_kind;
-    const LdsPlan lp = lds_plan(la.n, la.maps);
+    const LdsPlan lp = lds_plan(la.n, LD ? la.maps : (la.maps & ~MAP_LDIST), static_cast<uint32_t>(sizeof(B)));
+    if (LD && (lp.maps & MAP_LDIST) == 0) return hipErrorInvalidValue;
     k.bm_words = lp.bm_words;
     k.bd_words = lp.bd_words;
     k.bd_kind = lp.bd_kind;
+    k.ld_off = lp.ld_off;
     k.bd_inv_q = la.bd_inv_q;
     k.maps = static_cast<uint32_t>(lp.maps);
-    hipLaunchKernelGGL((batch_kernel<D, MODE, BS>), dim3(la.grid), dim3(BS), lp.bytes, stream, k);
+    hipLaunchKernelGGL((batch_kernel<D, MODE, BS, LD>), dim3(la.grid), dim3(BS), lp.bytes, stream, k);
     return hipGetLastError();
 }
 
-template <class D, int BS>
+template <class D, int BS, bool LD>
 hipError_t launch_mode(const LaunchArgs& a, hipStream_t stream) {
     switch (a.mode) {
-    case MODE_SSSP: return launch_typed<D, MODE_SSSP, BS>(a, stream);
-    case MODE_BC: return launch_typed<D, MODE_BC, BS>(a, stream);
-    case MODE_ARC1: return launch_typed<D, MODE_ARC1, BS>(a, stream);
-    case MODE_ARC1 | MODE_ARC2: return launch_typed<D, MODE_ARC1 | MODE_ARC2, BS>(a, stream);
+    case MODE_SSSP: return launch_typed<D, MODE_SSSP, BS, LD>(a, stream);
+    case MODE_BC: return launch_typed<D, MODE_BC, BS, LD>(a, stream);
+    case MODE_ARC1: return launch_typed<D, MODE_ARC1, BS, LD>(a, stream);
+    case MODE_ARC1 | MODE_ARC2: return launch_typed<D, MODE_ARC1 | MODE_ARC2, BS, LD>(a, stream);
     default: return hipErrorInvalidValue;
     }
 }
 
+// LDS distances are built for 256-lane workgroups only (small graphs run there)
 template <class D>
 hipError_t launch_block(const LaunchArgs& a, hipStream_t stream) {
     switch (a.block) {
-    case 256: return launch_mode<D, 256>(a, stream);
-    case 512: return launch_mode<D, 512>(a, stream);
-    case 1024: return launch_mode<D, 1024>(a, stream);
+    case 256: return (a.maps & MAP_LDIST) ? launch_mode<D, 256, true>(a, stream) : launch_mode<D, 256, false>(a, stream);
+    case 512: return launch_mode<D, 512, false>(a, stream);
+    case 1024: return launch_mode<D, 1024, false>(a, stream);
     default: return hipErrorInvalidValue;
     }
 }
 
-template <class D, int BS>
+template <class D, int BS, bool LD>
 int blocks_per_cu_bs(int mode, size_t dyn) {
     int nb = 0;
     hipError_t e = hipErrorInvalidValue;
     switch (mode) {
-    case MODE_SSSP: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<D, MODE_SSSP, BS>, BS, dyn); break;
-    case MODE_BC: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<D, MODE_BC, BS>, BS, dyn); break;
-    case MODE_ARC1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<D, MODE_ARC1, BS>, BS, dyn); break;
+    case MODE_SSSP: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<D, MODE_SSSP, BS, LD>, BS, dyn); break;
+    case MODE_BC: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<D, MODE_BC, BS, LD>, BS, dyn); break;
+    case MODE_ARC1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<D, MODE_ARC1, BS, LD>, BS, dyn); break;
     default:
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<D, MODE_ARC1 | MODE_ARC2, BS>, BS, dyn);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<D, MODE_ARC1 | MODE_ARC2, BS, LD>, BS, dyn);
         break;
     }
     return (e == hipSuccess && nb > 0) ? nb : 1;
@@ -1592,11 +1614,13 @@ int blocks_per_cu_bs(int mode, size_t dyn) {
 
 template <class D>
 int blocks_per_cu_mode(int mode, uint32_t n, int maps, int block) {
-    const size_t dyn = lds_plan(n, maps).bytes;
+    using B = typename DT<D>::B;
+    const bool ld = block == 256 && (maps & MAP_LDIST) != 0;
+    const size_t dyn = lds_plan(n, ld ? maps : (maps & ~MAP_LDIST), static_cast<uint32_t>(sizeof(B))).bytes;
     switch (block) {
-    case 512: return blocks_per_cu_bs<D, 512>(mode, dyn);
-    case 1024: return blocks_per_cu_bs<D, 1024>(mode, dyn);
-    default: return blocks_per_cu_bs<D, 256>(mode, dyn);
+    case 512: return blocks_per_cu_bs<D, 512, false>(mode, dyn);
+    case 1024: return blocks_per_cu_bs<D, 1024, false>(mode, dyn);
+    default: return ld ? blocks_per_cu_bs<D, 256, true>(mode, dyn) : blocks_per_cu_bs<D, 256, false>(mode, dyn);
     }
 }
 
@@ -1612,8 +1636,12 @@ hipError_t launch_batch(const LaunchArgs& a, hipStream_t stream) {
     return hipErrorInvalidValue;
 }
 
-int lds_plan_maps(uint32_t n, int maps) { return lds_plan(n, maps).maps; }
-int lds_plan_bound_kind(uint32_t n, int maps) { return static_cast<int>(lds_plan(n, maps).bd_kind); }
+int lds_plan_maps(DKind dk, uint32_t n, int maps) {
+    return lds_plan(n, maps, (dk == DKind::U32 || dk == DKind::F32) ? 4u : 8u).maps;
+}
+int lds_plan_bound_kind(DKind dk, uint32_t n, int maps) {
+    return static_cast<int>(lds_plan(n, maps, (dk == DKind::U32 || dk == DKind::F32) ? 4u : 8u).bd_kind);
+}
 
 int batch_blocks_per_cu(DKind dk, int mode, uint32_t n, int maps, int block) {
     switch (dk) {

@@ -250,11 +250,12 @@ def test_small_pile_forces_exact_bucket_advance(ctx, monkeypatch):
 
 
 @pytest.mark.parametrize("block", ["256", "1024"])
-@pytest.mark.parametrize("maps", [str(k) for k in range(16)])
+@pytest.mark.parametrize("maps", [str(k) for k in range(20)])
 def test_lds_bitmap_variants(ctx, monkeypatch, maps, block):
     """Every combination of the optional LDS maps (settle passes in discovery order or in increasing id;
     frontier de-duplication by global stamps or by an LDS bitmap; probes with no bounds, 16-bit bounds or, for
-    maps 8-15, the 4-bit distance codes) gives the oracle's distances bit-exactly and its betweenness within
+    maps 8-15, the 4-bit distance codes; for maps 16-19 in 256-lane workgroups, the distances themselves in LDS)
+    gives the oracle's distances bit-exactly and its betweenness within
     1e-9, for integer, FP32 and FP64 weights and both length kinds (|w| and 1/|w|), in 256-lane workgroups
     (plain arc loop) and 1024-lane ones (filter stage, survivors compacted by lane permutes)."""
     monkeypatch.setenv("NG_LDS_MAPS", maps)
