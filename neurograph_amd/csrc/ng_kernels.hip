@@ -399,8 +399,9 @@ __device__ __forceinline__ bool above_bound(typename T::B c, uint16_t bd) {
 // earlier code, which is still a valid (weaker) bound.
 __device__ __forceinline__ uint32_t code4(double x) { return x < 15.0 ? static_cast<uint32_t>(x) : 15u; }
 
-// Settle-pass form of the arc loop, for passes where few arcs survive the LDS filter (bitmap and bounds).
-// Stage A: UA arcs per lane get their owner, CSR entry and candidate, and are filtered. The survivors of the
+// Arc loop of the relaxation passes (settle passes and frontier phases) in wide workgroups, where few arcs
+// survive the LDS filter (bitmap and bounds). Stage A: UA arcs per lane get their owner, CSR entry and
+// candidate, and are filtered. The survivors of the
 // wave (ranked slot by slot from ballots alone) are moved to consecutive lanes by forward permutes, one full
 // permutation per slot (non-survivors fill the other lanes), so stage B (distance probe, update, apply)
 // runs once for up to 64 survivors instead of once per slot with a few live lanes. With more survivors than
