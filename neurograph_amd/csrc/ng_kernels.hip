@@ -664,7 +664,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
     // an earlier one). It replaces the global settled stamps, and in a settle pass a heavy arc (length >=
     // delta) into a registered node skips its global distance probe: the target's distance is final and
     // below hi while the candidate is at least hi, so the arc can neither improve nor be tight.
-    extern __shared__ uint32_t bm[];
+    // (16-byte aligned: the LDS distances start at an even word offset and are read as 8-byte values)
+    extern __shared__ __attribute__((aligned(16))) uint32_t bm[];
     const bool use_bm = a.bm_words != 0;
     // With `sorted`, nodes registered in the current bucket are also marked in bn; before the settle pass
     // the bucket's part of the settled list is written by enumerating bn, i.e. in increasing node id, so
