@@ -869,11 +869,11 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
             const B nd = T::add(dv, l);
             return (ok && !final_t && nd < cur) ? wg_min(DP(t), nd) : nd;
         };
-        // relaxation of arc e = (v, t) with length l from v at distance dv, given the probe (fin, cur) and the
-        // update (old); new frontier entries are stamped `nph`; in the settle pass the candidate tight arcs are
-        // logged
-        // (live: the arc was probed; nd: its candidate distance)
-        auto apply_core = [&](bool live, uint32_t e, uint32_t t, B nd, B cur, B old, uint32_t nph, bool log_it_if_close) {
+        // relaxation of arc e into t with candidate distance nd, given the probe (live: probed; cur) and the
+        // update (old); new frontier entries are stamped `nph`. In the betweenness modes every candidate that
+        // could be tight (nd <= cur) is logged: light arcs by the frontier phases (a node is expanded again after
+        // each improvement, so also with its final distance), heavy arcs by the settle passes.
+        auto apply_core = [&](bool live, uint32_t e, uint32_t t, B nd, B cur, B old, uint32_t nph) {
             bool pf = false, pp = false, parked = false;
             if (nd < old) {
                 ++c_improve;
@@ -903,7 +903,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
                 if (lane_id() == 0) atomicMin(&S.runmin, m);
             }
             if constexpr (MODE != MODE_SSSP) {
-                const bool lg = live && log_it_if_close && nd <= cur;
+                const bool lg = live && nd <= cur;
                 const unsigned long long mask = __ballot(lg);
                 if (mask != 0ull) {
                     const uint32_t lane = lane_id();
@@ -913,9 +913,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
                 }
             }
         };
-        auto apply = [&](bool ok, uint32_t, B dv, uint32_t e, uint32_t t, L l, bool final_t, B cur, B old, uint32_t nph,
-                         bool log_it_if_close) {
-            apply_core(ok && !final_t, e, t, T::add(dv, l), cur, old, nph, log_it_if_close);
+        auto apply = [&](bool ok, uint32_t, B dv, uint32_t e, uint32_t t, L l, bool final_t, B cur, B old, uint32_t nph) {
+            apply_core(ok && !final_t, e, t, T::add(dv, l), cur, old, nph);
         };
         // the same steps for the compacted settle pass, given the candidate: the filter (heavy arc into a
         // registered node, or candidate above the target's bound), the probe and the update
@@ -957,12 +956,12 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
                     expand_relax_compact<3, BS>(
                         Fc, nF, S, tgt, len, light_row, [](B dv, L l) -> B { return T::add(dv, l); }, skip_arc, probe_t,
                         lower_nd,
-                        [&](bool live, uint32_t e, uint32_t t, B nd, B cur, B old) { apply_core(live, e, t, nd, cur, old, nph, false); });
+                        [&](bool live, uint32_t e, uint32_t t, B nd, B cur, B old) { apply_core(live, e, t, nd, cur, old, nph); });
                 } else {
                     expand_relax<NG_UNROLL, BS>(
                         Fc, nF, S, tgt, len, light_row, probe, lower,
                         [&](bool ok, uint32_t v, B dv, uint32_t e, uint32_t t, L l, bool fin, B cur, B old) {
-                            apply(ok, v, dv, e, t, l, fin, cur, old, nph, false);
+                            apply(ok, v, dv, e, t, l, fin, cur, old, nph);
                         });
                 }
                 // first visit in this bucket -> settled list
@@ -995,7 +994,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
                 NG_TOCK(t_inner, 0);
             }
             if (aborted) break;
-            // ---- settle pass: every settled node's row once, with its final distance
+            // ---- settle pass: the heavy arcs of every settled node once, with its final distance
             const uint32_t nS = clampn(S.cnt[2]);
             if (aborted) break;
             if (sorted && nS > settle_from) {
@@ -1023,9 +1022,10 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
             if (nS > settle_from) {
                 NG_TICK(t_settle);
                 const uint32_t nph = phase + 1;
+                // heavy arcs only: the light ones were relaxed, and their candidates logged, by the phases
                 auto settle_row = [&](uint32_t v, B& dv, uint32_t& beg, uint32_t& cnt) {
                     dv = ldc(DP(v));
-                    beg = (MODE != MODE_SSSP) ? off[v] : lend[v];
+                    beg = lend[v];
                     cnt = off[v + 1] - beg;
                     c_settle += cnt;
                 };
@@ -1037,12 +1037,12 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
                     expand_relax_compact<kFilterArcs, BS>(
                         Sl + settle_from, nS - settle_from, S, tgt, len, settle_row,
                         [](B dv, L l) -> B { return T::add(dv, l); }, skip_arc, probe_t, lower_nd,
-                        [&](bool live, uint32_t e, uint32_t t, B nd, B cur, B old) { apply_core(live, e, t, nd, cur, old, nph, true); });
+                        [&](bool live, uint32_t e, uint32_t t, B nd, B cur, B old) { apply_core(live, e, t, nd, cur, old, nph); });
                 } else {
                     expand_relax<NG_UNROLL, BS>(
                         Sl + settle_from, nS - settle_from, S, tgt, len, settle_row, probe, lower,
                         [&](bool ok, uint32_t v, B dv, uint32_t e, uint32_t t, L l, bool fin, B cur, B old) {
-                            apply(ok, v, dv, e, t, l, fin, cur, old, nph, true);
+                            apply(ok, v, dv, e, t, l, fin, cur, old, nph);
                         });
                 }
                 bsync();
@@ -1217,10 +1217,11 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
             // arc e into t is tight when its candidate c equals the final distance of t
             uint32_t n_tight = 0;
             auto tight = [&](uint32_t t, B c) -> bool { return !bd_skip(t, c) && c == ldc(DP(t)); };
+            // the frontier phases may log a light arc more than once: an arc counts when its bit is new
             auto mark = [&](uint32_t t, uint32_t e, B c) {
                 if (tight(t, c)) {
-                    wg_or(&bits[e >> 5], 1u << (e & 31));
-                    ++n_tight;
+                    const uint32_t bit = 1u << (e & 31);
+                    if ((wg_or(&bits[e >> 5], bit) & bit) == 0u) ++n_tight;
                 }
             };
             // every candidate (from the log, or from the full rows when the log overflowed) through fn
@@ -1257,10 +1258,17 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 
             // Brandes on a tree pushes dependencies to parents: every delta is a subtree size, an integer, so the
             // atomic sums are exact in any order. The sigma layers record each node's parent and clear the bitmap.
             const bool push_dep = MODE == MODE_BC && tree;
-            if (!tree) {
-                each_candidate([&](uint32_t t, uint32_t e, B c) {
-                    if (tight(t, c)) wg_add(&R[t].indeg, 1u);
-                });
+            if (!tree) {  // tight in-degrees from the bitmap rows (each tight arc once)
+                for (uint32_t i = tid; i < nS; i += BS) {
+                    const uint32_t u = Sl[i];
+                    for_row_words(bits, off[u], off[u + 1], [&](uint32_t wi, uint32_t w) {
+                        while (w) {
+                            const uint32_t e = (wi << 5) + static_cast<uint32_t>(__ffs(w) - 1);
+                            w &= w - 1;
+                            wg_add(&R[tgt[e]].indeg, 1u);
+                        }
+                    });
+                }
                 bsync();
             }
             NG_TOCK(t_verify, 4);
