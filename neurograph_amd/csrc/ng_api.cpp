@@ -313,8 +313,8 @@ int lds_maps_for(DKind dk, int mode, uint32_t n, int block, bool allow4) {
     const int b = allow4 ? (MAP_BOUND | MAP_BOUND4) : static_cast<int>(MAP_BOUND);
     if (const char* e = std::getenv("NG_LDS_MAPS")) return std::atoi(e) & (allow4 ? 31 : 23);
     const int base = batch_blocks_per_cu(dk, mode, n, 0, block);
-    // small graphs in 256-lane workgroups: the distances themselves in LDS
-    if (block == 256)
+    // graphs small enough: the distances themselves in LDS (256- and 1024-lane workgroups)
+    if (block == 256 || block == 1024)
         for (int maps : {MAP_LDIST | MAP_SORTED | MAP_FRONTIER, MAP_LDIST | MAP_FRONTIER, MAP_LDIST | MAP_SORTED, static_cast<int>(MAP_LDIST)})
             if (lds_plan_maps(dk, n, maps) == maps && batch_blocks_per_cu(dk, mode, n, maps, block) >= base) return maps;
     // 16-bit bounds where they fit, else 4-bit codes (also where 16-bit bounds would cost workgroups: on a

@@ -1596,13 +1596,14 @@ hipError_t launch_mode(const LaunchArgs& a, hipStream_t stream) {
     }
 }
 
-// LDS distances are built for 256-lane workgroups only (small graphs run there)
+// LDS distances are built for 256-lane workgroups (small graphs) and 1024-lane ones (one workgroup per CU
+// holds up to ~110 KB of distances)
 template <class D>
 hipError_t launch_block(const LaunchArgs& a, hipStream_t stream) {
     switch (a.block) {
     case 256: return (a.maps & MAP_LDIST) ? launch_mode<D, 256, true>(a, stream) : launch_mode<D, 256, false>(a, stream);
     case 512: return launch_mode<D, 512, false>(a, stream);
-    case 1024: return launch_mode<D, 1024, false>(a, stream);
+    case 1024: return (a.maps & MAP_LDIST) ? launch_mode<D, 1024, true>(a, stream) : launch_mode<D, 1024, false>(a, stream);
     default: return hipErrorInvalidValue;
     }
 }
@@ -1625,11 +1626,11 @@ int blocks_per_cu_bs(int mode, size_t dyn) {
 template <class D>
 int blocks_per_cu_mode(int mode, uint32_t n, int maps, int block) {
     using B = typename DT<D>::B;
-    const bool ld = block == 256 && (maps & MAP_LDIST) != 0;
+    const bool ld = (block == 256 || block == 1024) && (maps & MAP_LDIST) != 0;
     const size_t dyn = lds_plan(n, ld ? maps : (maps & ~MAP_LDIST), static_cast<uint32_t>(sizeof(B))).bytes;
     switch (block) {
     case 512: return blocks_per_cu_bs<D, 512, false>(mode, dyn);
-    case 1024: return blocks_per_cu_bs<D, 1024, false>(mode, dyn);
+    case 1024: return ld ? blocks_per_cu_bs<D, 1024, true>(mode, dyn) : blocks_per_cu_bs<D, 1024, false>(mode, dyn);
     default: return ld ? blocks_per_cu_bs<D, 256, true>(mode, dyn) : blocks_per_cu_bs<D, 256, false>(mode, dyn);
     }
 }
