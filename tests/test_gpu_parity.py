@@ -356,3 +356,37 @@ def test_autotune_shape(ctx):
     srcs = np.arange(0, 5000, 41)
     g.brandes(srcs)
     assert g.last_stats()["block"] == b and g.last_stats()["delta"] == d
+
+
+def test_random_configurations(ctx, monkeypatch):
+    """30 seeded random cases over graph size and density (uniform or local arcs), weight kind (small and
+    2^20..2^31 integers, FP32, FP64), length kind and workgroup size, with the library's own LDS map choice
+    (bitmaps, 16-bit bounds, 4-bit codes or LDS distances): distances bit-exact, betweenness within 1e-9."""
+    import neurograph_amd as ng
+
+    rng = np.random.default_rng(2024)
+    for case in range(30):
+        n = int(rng.choice([60, 300, 2000, 9000]))
+        m = max(1, int(n * float(rng.choice([1.5, 4, 12, 40]))))
+        kind = str(rng.choice(["i32s", "i32l", "f32", "f64"]))
+        block = str(rng.choice([256, 512, 1024]))
+        lkind = int(rng.choice([orc.L_ABS, orc.L_INVERSE]))
+        src = rng.integers(0, n, m)
+        dst = rng.integers(0, n, m) if rng.random() < 0.5 else (src + rng.integers(1, max(2, n // 50), m)) % n
+        if kind == "i32s":
+            w = rng.integers(1, 10, m).astype(np.int32)
+        elif kind == "i32l":
+            w = rng.integers(1 << 20, 2**31 - 1, m).astype(np.int32)
+        else:
+            w = np.exp(rng.normal(0.0, 0.5, m)).astype(np.float32 if kind == "f32" else np.float64)
+        c = orc.Csr.from_edges(n, src, dst, w, dtype=w.dtype)
+        srcs = np.unique(rng.integers(0, n, 16)).astype(np.uint32)
+        monkeypatch.setenv("NG_BLOCK_SIZE", block)
+        g = ng.Graph(ctx, ng.HostGraph(c.n, c.off, c.tgt, c.w))
+        ref = np.stack([orc.sssp(c, int(s), lkind)[0] for s in srcs])
+        _, dist = g.sssp(srcs, lkind=lkind, want_dist=True)
+        np.testing.assert_array_equal(dist, ref, err_msg=f"case {case}: n={n} m={m} {kind} block {block} lkind {lkind}")
+        if lkind == orc.L_ABS:
+            np.testing.assert_allclose(g.brandes(srcs), orc.brandes(c, sources=srcs)["bc"], rtol=1e-9, atol=1e-9,
+                                       err_msg=f"case {case}")
+        g.close()
