@@ -284,7 +284,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--delta", type=float, default=0.0, help="bucket width (0: autotune width and lanes per source)")
-    ap.add_argument("--block", type=int, default=0, choices=[0, 256, 512, 1024],
+    ap.add_argument("--block", type=int, default=0, choices=[0, 64, 256, 512, 1024],
                     help="lanes per source workgroup when --delta is given (0: the library default)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (a known-size kernel for scaling profiler byte counters)")

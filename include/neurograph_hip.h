@@ -157,7 +157,8 @@ ng_status ng_apsp_adjoint(ng_graph* g, const uint32_t* sources, uint32_t n_sourc
 ng_status ng_autotune_delta(ng_graph* g, ng_length_kind lkind, int mode, const uint32_t* sample, uint32_t n_sample,
                             double* delta_out);
 
-/* Picks the bucket width and the lanes per workgroup (256, 512 or 1024) together for (lkind, mode) by timing
+/* Picks the bucket width and the lanes per workgroup (64 -- one wavefront per source, tried on graphs of at most
+ * 32768 nodes --, 256, 512 or 1024) together for (lkind, mode) by timing
  * the batched kernel on `sample` sources (widths: the 5 % length quantile times 1/4 ... 8 in powers of two,
  * then the best one times 1/sqrt(2) and sqrt(2)), and stores both in the graph (used whenever delta <= 0). Fewer,
  * wider workgroups keep fewer sources in flight, so their per-source state stays cache-resident on large

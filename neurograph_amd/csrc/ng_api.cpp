@@ -329,11 +329,11 @@ int lds_maps_for(DKind dk, int mode, uint32_t n, int block, bool allow4) {
 
 // Lanes per workgroup for a call: explicit, else tuned, else NG_BLOCK_SIZE (experiments), else the default.
 int pick_block(const View& v, int mode_idx, int requested) {
-    if (requested == 256 || requested == 512 || requested == 1024) return requested;
+    if (requested == 64 || requested == 256 || requested == 512 || requested == 1024) return requested;
     if (v.tuned_block[mode_idx] > 0) return v.tuned_block[mode_idx];
     if (const char* e = std::getenv("NG_BLOCK_SIZE")) {
         const int b = std::atoi(e);
-        if (b == 256 || b == 512 || b == 1024) return b;
+        if (b == 64 || b == 256 || b == 512 || b == 1024) return b;
     }
     return kBlock;
 }
@@ -370,8 +370,11 @@ uint32_t wanted_pcap(uint32_t n) {
 }
 
 // Makes sure workspace w holds at least `slots` slots for graphs of up to n nodes and m arcs, with every
-// array `mode` needs and dist in kind dk. Growing keeps the larger of the old and new dimensions.
-ng_status ensure_ws(Workspace& w, hipStream_t st, uint32_t n, uint64_t m, uint32_t slots, int mode, DKind dk, uint32_t n_sources) {
+// array `mode` needs and dist in kind dk. Growing keeps the larger of the old and new dimensions, except that
+// old slots beyond `slot_cap` (the caller's memory budget at the new dimensions) are dropped: many small
+// slots of a small graph must not be carried over to a large one.
+ng_status ensure_ws(Workspace& w, hipStream_t st, uint32_t n, uint64_t m, uint32_t slots, int mode, DKind dk, uint32_t n_sources,
+                    uint32_t slot_cap) {
     int need_bits = (mode == MODE_SSSP) ? 0 : (MODE_BC | MODE_ARC1 | MODE_ARC2);
     const bool realloc = w.slots < slots || w.n < n || w.m < m || dk_size(dk) > w.dist_bytes || (need_bits & ~w.mode_bits) != 0 ||
                          w.pcap != wanted_pcap(std::max(w.n, n));
@@ -379,7 +382,7 @@ ng_status ensure_ws(Workspace& w, hipStream_t st, uint32_t n, uint64_t m, uint32
         if (w.dist != nullptr) {
             n = std::max(n, w.n);
             m = std::max(m, w.m);
-            slots = std::max(slots, w.slots);
+            slots = std::max(slots, std::min(w.slots, slot_cap));
             n_sources = std::max(n_sources, w.summaries_cap);
             need_bits |= w.mode_bits;
         }
@@ -509,8 +512,10 @@ ng_status run_batch(ng_graph* g, const RunSpec& rs, uint32_t* slots_used, double
     // memory already held by the context's workspace counts as available
     const uint64_t held = static_cast<uint64_t>(w.slots) * slot_bytes(w.n, w.m, rs.mode, log_capacity(w.n, w.m));
     const uint64_t budget = static_cast<uint64_t>(static_cast<double>(free_b + held) * 0.6);
-    if (per > 0) slots = std::min<uint64_t>(slots, std::max<uint64_t>(1, budget / per));
-    st = ensure_ws(w, ctx->stream, g->n, g->m, static_cast<uint32_t>(slots), rs.mode, v.dk, rs.n_sources);
+    const uint64_t slot_cap = per > 0 ? std::max<uint64_t>(1, budget / per) : slots;
+    slots = std::min<uint64_t>(slots, slot_cap);
+    st = ensure_ws(w, ctx->stream, g->n, g->m, static_cast<uint32_t>(slots), rs.mode, v.dk, rs.n_sources,
+                   static_cast<uint32_t>(std::min<uint64_t>(slot_cap, 0xFFFFFFFFu)));
     if (st != NG_OK) return st;
     const hipStream_t s = ctx->stream;
     if (rs.n_sources) NG_HIP(hipMemcpyAsync(w.sources, rs.sources, sizeof(uint32_t) * rs.n_sources, hipMemcpyHostToDevice, s));
@@ -971,7 +976,8 @@ ng_status ng_autotune(ng_graph* g, ng_length_kind lkind, int mode, const uint32_
     }
     double best_d = cand.front(), best_ms = std::numeric_limits<double>::infinity();
     int best_b = kBlock;
-    for (int b : {256, 512, 1024}) {
+    for (int b : {64, 256, 512, 1024}) {
+        if (b == 64 && g->n > kWaveSourceMaxNodes) continue;
         bool first = true;
         for (double d : cand) {
             for (int rep = first ? 0 : 1; rep < 2; ++rep) {  // one untimed run per block size (workspace, caches)
@@ -1141,7 +1147,7 @@ ng_status cohort_build(ng_cohort* c) {
     if (s1 != NG_OK) return s1;
     NG_HIP(hipMalloc(&c->d_flush, 8 * std::max<uint64_t>(F, 1)));
     NG_HIP(hipMalloc(&c->d_bc, 8 * std::max<uint32_t>(c->total, 1)));
-    ng_status s2 = ensure_ws(w, st, c->n_max, c->m_max, c->slots, MODE_BC, c->dk, c->total);
+    ng_status s2 = ensure_ws(w, st, c->n_max, c->m_max, c->slots, MODE_BC, c->dk, c->total, c->slots);
     if (s2 != NG_OK) return s2;
     NG_HIP(hipStreamSynchronize(st));
     c->ws_generation = w.generation;
@@ -1191,7 +1197,7 @@ ng_status cohort_replay(ng_cohort* c) {
     // the context workspace is shared: plain calls in between may have grown it (new buffers: re-capture) or
     // left +infinity of another distance kind in it (refilled here, outside the capture)
     if (c->built) {
-        ng_status s0 = ensure_ws(ctx->ws, st, c->n_max, c->m_max, c->slots, MODE_BC, c->dk, c->total);
+        ng_status s0 = ensure_ws(ctx->ws, st, c->n_max, c->m_max, c->slots, MODE_BC, c->dk, c->total, c->slots);
         if (s0 != NG_OK) return s0;
         if (ctx->ws.generation != c->ws_generation) c->built = false;
     }

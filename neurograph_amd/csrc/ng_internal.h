@@ -12,9 +12,12 @@
 namespace ng {
 
 // Lanes per workgroup (one source slot, the frontier tile size): the default and the largest supported; the
-// batched kernel is built for 256, 512 and 1024.
+// batched kernel is built for 64 (one wavefront per source), 256, 512 and 1024.
 constexpr int kBlock = 256;
 constexpr int kMaxBlock = 1024;
+// Largest graph for which the autotuner also times one wavefront per source: smaller graphs' frontiers rarely
+// fill wider workgroups, while larger ones would hold thousands of slots of per-node state.
+constexpr uint32_t kWaveSourceMaxNodes = 32768;
 
 // Device distance element type: lengths and distances are kept in one of these.
 enum class DKind : int { U32 = 0, U64 = 1, F32 = 2, F64 = 3 };
@@ -133,7 +136,7 @@ struct LaunchArgs {
     int dist_out_kind;        // ng_dist_kind
     int maps;                 // extra LDS maps: MAP_SORTED (settle passes in increasing id), MAP_FRONTIER, MAP_BOUND(4)
     double bd_inv_q = 0.0;    // scale of the 4-bit distance codes (MAP_BOUND4)
-    int block = kBlock;       // lanes per workgroup: 256, 512 or 1024
+    int block = kBlock;       // lanes per workgroup: 64, 256, 512 or 1024
     // cohort launch (gtab != nullptr; view, sources and n_sources unused; n = the largest member's n)
     const CohortGraph* gtab = nullptr;
     uint32_t n_graphs = 0;

@@ -646,14 +646,15 @@ __device__ __forceinline__ void for_row_words(const uint32_t* bits, uint32_t b, 
 
 // Resident waves per SIMD the hot kernel is compiled for (the register budget follows: 512 / waves VGPRs).
 // The path is bound by the latency of random accesses, so occupancy matters more than the few registers
-// spilled to fit. Workgroups of 512 or 1024 lanes fill a SIMD with 4 waves from one or two workgroups.
+// spilled to fit. Workgroups of 512 or 1024 lanes fill a SIMD with 4 waves from one or two workgroups; 64-lane
+// workgroups (one wavefront per source, for small graphs) take the 256-lane kernel's register budget.
 #ifndef NG_WAVES_PER_EU
 #define NG_WAVES_PER_EU 5
 #endif
 // LD: the current source's distances live in LDS (graphs small enough; 256-lane workgroups), so probes and
 // updates are LDS operations; otherwise they live in the slot's global array.
 template <class D, int MODE, int BS, bool LD>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? NG_WAVES_PER_EU : 4))) void batch_kernel(
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? NG_WAVES_PER_EU : 4))) void batch_kernel(
     const KArgs<D> a) {
     using T = DT<D>;
     using B = typename T::B;
@@ -1597,10 +1598,12 @@ hipError_t launch_mode(const LaunchArgs& a, hipStream_t stream) {
 }
 
 // LDS distances are built for 256-lane workgroups (small graphs) and 1024-lane ones (one workgroup per CU
-// holds up to ~110 KB of distances)
+// holds up to ~110 KB of distances); 64-lane workgroups (one wavefront per source) keep them in the slot's
+// global array, since a few KB of distances per wave would already cut the resident waves
 template <class D>
 hipError_t launch_block(const LaunchArgs& a, hipStream_t stream) {
     switch (a.block) {
+    case 64: return launch_mode<D, 64, false>(a, stream);
     case 256: return (a.maps & MAP_LDIST) ? launch_mode<D, 256, true>(a, stream) : launch_mode<D, 256, false>(a, stream);
     case 512: return launch_mode<D, 512, false>(a, stream);
     case 1024: return (a.maps & MAP_LDIST) ? launch_mode<D, 1024, true>(a, stream) : launch_mode<D, 1024, false>(a, stream);
@@ -1629,6 +1632,7 @@ int blocks_per_cu_mode(int mode, uint32_t n, int maps, int block) {
     const bool ld = (block == 256 || block == 1024) && (maps & MAP_LDIST) != 0;
     const size_t dyn = lds_plan(n, ld ? maps : (maps & ~MAP_LDIST), static_cast<uint32_t>(sizeof(B))).bytes;
     switch (block) {
+    case 64: return blocks_per_cu_bs<D, 64, false>(mode, dyn);
     case 512: return blocks_per_cu_bs<D, 512, false>(mode, dyn);
     case 1024: return ld ? blocks_per_cu_bs<D, 1024, true>(mode, dyn) : blocks_per_cu_bs<D, 1024, false>(mode, dyn);
     default: return ld ? blocks_per_cu_bs<D, 256, true>(mode, dyn) : blocks_per_cu_bs<D, 256, false>(mode, dyn);

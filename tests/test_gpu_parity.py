@@ -249,15 +249,15 @@ def test_small_pile_forces_exact_bucket_advance(ctx, monkeypatch):
         np.testing.assert_allclose(bc, orc.brandes(c, sources=srcs)["bc"], rtol=1e-9, atol=1e-9)
 
 
-@pytest.mark.parametrize("block", ["256", "1024"])
+@pytest.mark.parametrize("block", ["64", "256", "1024"])
 @pytest.mark.parametrize("maps", [str(k) for k in range(20)])
 def test_lds_bitmap_variants(ctx, monkeypatch, maps, block):
     """Every combination of the optional LDS maps (settle passes in discovery order or in increasing id;
     frontier de-duplication by global stamps or by an LDS bitmap; probes with no bounds, 16-bit bounds or, for
     maps 8-15, the 4-bit distance codes; for maps 16-19 in 256-lane workgroups, the distances themselves in LDS)
     gives the oracle's distances bit-exactly and its betweenness within
-    1e-9, for integer, FP32 and FP64 weights and both length kinds (|w| and 1/|w|), in 256-lane workgroups
-    (plain arc loop) and 1024-lane ones (filter stage, survivors compacted by lane permutes)."""
+    1e-9, for integer, FP32 and FP64 weights and both length kinds (|w| and 1/|w|), in 64- and 256-lane
+    workgroups (plain arc loop) and 1024-lane ones (filter stage, survivors compacted by lane permutes)."""
     monkeypatch.setenv("NG_LDS_MAPS", maps)
     monkeypatch.setenv("NG_BLOCK_SIZE", block)
     # int32 weights up to 2^31 - 1 take the 64-bit integer distances, whose bounds round
@@ -325,9 +325,10 @@ def test_distance_histogram_and_matrix(ctx, wdtype):
     assert int(counts.sum()) == len(srcs) * (hg.n - 1) - int(summ["n_unreached"].sum())
 
 
-@pytest.mark.parametrize("block", ["256", "512", "1024"])
+@pytest.mark.parametrize("block", ["64", "256", "512", "1024"])
 def test_workgroup_sizes(ctx, monkeypatch, block):
-    """Every workgroup size the kernel is built for (one source per workgroup of 256, 512 or 1024 lanes) gives
+    """Every workgroup size the kernel is built for (one source per wavefront, or per workgroup of 256, 512 or
+    1024 lanes) gives
     the oracle's distances bit-exactly and its betweenness and edge path counts within 1e-9."""
     import neurograph_amd as ng
 
@@ -352,7 +353,7 @@ def test_autotune_shape(ctx):
     c = rand_graph(5000, 60000, 21, wdtype=np.float32)
     g = dev_graph(ctx, c)
     d, b = g.autotune_shape(np.arange(0, 5000, 2), mode=1)
-    assert d > 0 and b in (256, 512, 1024)
+    assert d > 0 and b in (64, 256, 512, 1024)
     srcs = np.arange(0, 5000, 41)
     g.brandes(srcs)
     assert g.last_stats()["block"] == b and g.last_stats()["delta"] == d
