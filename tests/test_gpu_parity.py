@@ -391,3 +391,26 @@ def test_random_configurations(ctx, monkeypatch):
             np.testing.assert_allclose(g.brandes(srcs), orc.brandes(c, sources=srcs)["bc"], rtol=1e-9, atol=1e-9,
                                        err_msg=f"case {case}")
         g.close()
+
+
+@pytest.mark.parametrize("block", ["64", "256", "1024"])
+def test_many_sources_per_slot(ctx, monkeypatch, block):
+    """Few slots, so each runs dozens of sources one after another: every source must find its slot restored
+    (distances, records, bitmaps, lists) by the previous one, whether the previous source's shortest paths
+    formed a tree (continuous weights: the tree layers restore the nodes) or not (small integer weights: ties),
+    or a mix (wide integer weights). Distances bit-exact, betweenness and edge path counts within 1e-9."""
+    monkeypatch.setenv("NG_MAX_SLOTS", "3")
+    monkeypatch.setenv("NG_BLOCK_SIZE", block)
+    for wdtype, seed, wlo, whi in ((np.float64, 51, 1, 9), (np.int32, 52, 1 << 20, 2**31 - 1), (np.int32, 53, 1, 4),
+                                   (np.float32, 54, 1, 9)):
+        c = rand_graph(1500, 12000, seed, wdtype=wdtype, wlo=wlo, whi=whi)
+        g = dev_graph(ctx, c)
+        srcs = np.arange(0, 1500, 7)
+        _, dist = g.sssp(srcs, want_dist=True)
+        np.testing.assert_array_equal(dist, np.stack([orc.sssp(c, int(s))[0] for s in srcs]))
+        assert g.last_stats()["slots"] == 3
+        ref = orc.brandes(c, sources=srcs, edge=True)
+        np.testing.assert_allclose(g.brandes(srcs), ref["bc"], rtol=1e-9, atol=1e-9)
+        np.testing.assert_allclose(g.edge_dependency(srcs), ref["edge"], rtol=1e-9, atol=1e-12)
+        _, dist2 = g.sssp(srcs, want_dist=True)  # the slots are clean after Brandes too
+        np.testing.assert_array_equal(dist2, dist)
