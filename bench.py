@@ -290,8 +290,8 @@ def main():
                     help="also run one device copy of this many MiB (a known-size kernel for scaling profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (sources would be split across GPUs with "
-              "one final reduction of the betweenness vector; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: this library has no multi-GPU path (sources would be split across GPUs with "
+              "one final all-reduce of the n-entry betweenness vector and of the scalar sums)", file=sys.stderr)
         sys.exit(2)
 
     if args.block:

@@ -153,7 +153,7 @@ ng_status ng_apsp_adjoint(ng_graph* g, const uint32_t* sources, uint32_t n_sourc
 
 /* Picks the bucket width for (lkind, mode) by timing candidate widths on `sample` sources and stores it
  * in the graph (used whenever delta <= 0). mode: 0 = SSSP, 1 = Brandes. The reference takes a fixed
- * integer --apsp-delta (neurograph.cpp:515); tuning is this build's extension. */
+ * integer --apsp-delta (neurograph.cpp:515); tuning is an extension of this library. */
 ng_status ng_autotune_delta(ng_graph* g, ng_length_kind lkind, int mode, const uint32_t* sample, uint32_t n_sample,
                             double* delta_out);
 

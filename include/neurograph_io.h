@@ -36,7 +36,7 @@ void ng_host_csr_free(ng_host_csr* g);
  * then the transforms of DistributedGraph::construct_graph (DistributedGraph.cpp:85-101) and the upload
  * rule that skips weight-0 arcs (DistributedGraph.cpp:222-225). n_ranks = 0 detects the rank count from
  * the positions files. wkind NG_W_I32 parses weights exactly like the reference (istream >> int32, so
- * "1.5" reads as 1); NG_W_F32 / NG_W_F64 parse real-valued weights (this build's extension). */
+ * "1.5" reads as 1); NG_W_F32 / NG_W_F64 parse real-valued weights (an extension of this library). */
 ng_status ng_load_graph_dir(const char* dir, const char* prefix, int n_ranks, int remove_self_arcs, int undirected,
                             int all_weights_one, ng_weight_kind wkind, ng_host_csr* out);
 

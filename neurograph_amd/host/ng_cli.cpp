@@ -102,7 +102,7 @@ int main(int argc, char** argv) {
         else if (f == "avg-apsp-inv") apsp_inv = true;
         else if (f == "avg-centrality") centrality = true;
         else {
-            std::cerr << "The argument \"" << f << "\" is not a valid algorithm of this build (avg-apsp, avg-apsp-inv, avg-centrality)\n";
+            std::cerr << "The argument \"" << f << "\" is not a valid algorithm of this implementation (avg-apsp, avg-apsp-inv, avg-centrality)\n";
             return EXIT_FAILURE;
         }
     }

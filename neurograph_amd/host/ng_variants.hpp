@@ -114,7 +114,7 @@ public:
         if (delta == 0) throw Exception("delta-stepping needs a positive delta");
         return aggregate_integer(run(graph, NG_LEN_RAW, static_cast<double>(delta)), graph.number_nodes());
     }
-    // The same aggregation for real-valued weights (this build's extension).
+    // The same aggregation for real-valued weights (an extension of this library).
     static ApspGlobalResult<double> compute_apsp_real(const Graph& graph, double delta = 0.0) {
         return aggregate_real(run(graph, NG_LEN_ABS, delta), graph.number_nodes());
     }

@@ -1,6 +1,6 @@
 // ORACLE — test infrastructure only. Nothing in the product links, loads or calls this file.
-// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it, and only as the checker
-// (or the timed CPU baseline), never as the thing that is measured or shipped.
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it, and only as the reference
+// check (or the timed CPU baseline), never as the thing that is measured or shipped.
 //
 // A sequential CPU restatement of neurograph's APSP / Brandes hot path, generic in the weight type
 // (the reference is integer-only: weight_type = int32, distance_type = uint64, Types.h:31-33).
@@ -62,7 +62,7 @@ struct Csr {
 
 // DijkstraCommon::transform_weight, DijkstraCommon.h:36-54.
 //   |w| (widened before abs, :41-43) for the plain metric; 1/|w| in double with 0 -> max for the inverse metric.
-// The real-valued weight kinds are this build's extension: |w| keeps the weight's own precision.
+// The real-valued weight kinds are an extension of this library: |w| keeps the weight's own precision.
 template <class D, class W>
 inline D transform_weight(W w, int lkind) {
     if (lkind == L_INVERSE) {

@@ -1,7 +1,7 @@
 """ORACLE -- test infrastructure only.
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg may import this
-module, and only as the checker (or as the timed CPU baseline). The product (``neurograph_amd``)
+module, and only as the reference check (or as the timed CPU baseline). The product (``neurograph_amd``)
 never imports, loads or calls anything under ``oracle/``.
 
 ctypes front end of ``oracle/oracle.cpp``, a CPU restatement of neurograph's APSP / Brandes hot path
@@ -25,7 +25,7 @@ function                    restates                                       pinne
                                                                             force on tiny graphs; cycle closed form;
                                                                             invariants sum BC = sum (d-1) (unit w)
 ``brandes(seed=1)``         the 1/d^2-seeded dependency (adjoint of the     central finite differences of the
-                            average efficiency, this build's extension)    oracle's own APSP (FP64, h=1e-6);
+                            average efficiency, a library extension)        oracle's own APSP (FP64, h=1e-6);
                                                                             homogeneity sum_e w_e dE/dw_e = -E
 ``delta_stepping``          DeltaStepping.h:36-82, EpochCommon.h:46-49,     reference per-source tables for every
                             PushCheckVariant.h:44-126                       delta in {1,4,10,10000}; equality with

@@ -45,7 +45,7 @@ def test_host_side_argument_errors_need_no_gpu():
     assert b"context" in L.ng_last_error()
     assert L.ng_sssp_summaries(None, None, 0, 0, 0.0, None, None) == _lib.NG_EINVAL
     st = L.ng_context_create(0, ctypes.byref(h))
-    if st == _lib.NG_OK:  # a GPU is present (the GPU box): clean up
+    if st == _lib.NG_OK:  # a GPU is present: clean up
         L.ng_context_destroy(h)
     else:
         assert st in (_lib.NG_EDEVICE, _lib.NG_EINVAL)
