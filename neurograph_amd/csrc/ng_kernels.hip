@@ -1287,19 +1287,49 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                 ++nl;
                 for (uint32_t i = lb + tid; i < le; i += BS) {
                     const uint32_t u = order[i];
-                    const double su = tree ? 1.0 : ldc(&R[u].sigma);
+                    if (tree) {
+                        // a tree node releases each tight child: up to four children's targets are loaded
+                        // together and appended with one LDS atomic
+                        uint32_t e0 = 0, e1 = 0, e2 = 0, e3 = 0, k = 0;
+                        auto flush = [&]() {
+                            const uint32_t t0 = tgt[e0];
+                            const uint32_t t1 = k > 1 ? tgt[e1] : 0u;
+                            const uint32_t t2 = k > 2 ? tgt[e2] : 0u;
+                            const uint32_t t3 = k > 3 ? tgt[e3] : 0u;
+                            const uint32_t pos = atomicAdd(&S.cnt[4], k);
+                            auto put = [&](uint32_t j, uint32_t t) {
+                                if (push_dep) stc(&R[t].spare, u);
+                                if (pos + j < n) order[pos + j] = t;
+                            };
+                            put(0, t0);
+                            if (k > 1) put(1, t1);
+                            if (k > 2) put(2, t2);
+                            if (k > 3) put(3, t3);
+                            k = 0;
+                        };
+                        for_row_words(bits, off[u], off[u + 1], [&](uint32_t wi, uint32_t w) {
+                            if (push_dep) wg_and(&bits[wi], ~w);
+                            while (w) {
+                                const uint32_t e = (wi << 5) + static_cast<uint32_t>(__ffs(w) - 1);
+                                w &= w - 1;
+                                e0 = k == 0 ? e : e0;
+                                e1 = k == 1 ? e : e1;
+                                e2 = k == 2 ? e : e2;
+                                e3 = k == 3 ? e : e3;
+                                if (++k == 4u) flush();
+                            }
+                        });
+                        if (k != 0u) flush();
+                        continue;
+                    }
+                    const double su = ldc(&R[u].sigma);
                     for_row_words(bits, off[u], off[u + 1], [&](uint32_t wi, uint32_t w) {
-                        if (push_dep) wg_and(&bits[wi], ~w);
                         while (w) {
                             const uint32_t e = (wi << 5) + static_cast<uint32_t>(__ffs(w) - 1);
                             w &= w - 1;
                             const uint32_t t = tgt[e];
-                            if (push_dep) stc(&R[t].spare, u);
-                            bool released = true;  // a tree node's only tight in-arc releases it
-                            if (!tree) {
-                                wg_add(&R[t].sigma, su);
-                                released = wg_sub(&R[t].indeg, 1u) == 1u;
-                            }
+                            wg_add(&R[t].sigma, su);
+                            const bool released = wg_sub(&R[t].indeg, 1u) == 1u;
                             if (released) {
                                 const uint32_t pos = atomicAdd(&S.cnt[4], 1u);
                                 if (pos < n) order[pos] = t;
