@@ -1376,24 +1376,53 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                         const uint32_t u = order[i];
                         const double su = tree ? 1.0 : ldc(&R[u].sigma);
                         double acc1 = 0.0, acc2 = 0.0;
+                        // the tight children in increasing arc order, two at a time: both children's loads are
+                        // in flight together, and the sums still add the terms in arc order
+                        auto visit = [&](uint32_t ea, uint32_t eb, bool two) {
+                            const uint32_t ta = tgt[ea];
+                            const uint32_t tb = two ? tgt[eb] : ta;
+                            const double sta = tree ? 1.0 : ldc(&R[ta].sigma);
+                            const double stb = tree ? 1.0 : ldc(&R[tb].sigma);
+                            const double da = ldc(&R[ta].delta1);
+                            const double db = ldc(&R[tb].delta1);
+                            uint32_t pa = 0, pb = 0;
+                            if constexpr ((MODE & MODE_ARC1) != 0) {
+                                pa = perm[ea];
+                                pb = two ? perm[eb] : pa;
+                            }
+                            double dta = 0.0, dtb = 0.0, xa = 0.0, xb = 0.0;
+                            if constexpr ((MODE & MODE_ARC2) != 0) {
+                                dta = T::val(ldc(DP(ta)));
+                                dtb = T::val(ldc(DP(tb)));
+                                xa = ldc(&delta2[ta]);
+                                xb = ldc(&delta2[tb]);
+                            }
+                            auto term = [&](uint32_t pe, double st, double dl, double dt, double x) {
+                                const double term1 = su * ((1.0 + dl) / st);
+                                acc1 += term1;
+                                if constexpr ((MODE & MODE_ARC1) != 0) unsafeAtomicAdd(&a.grad1[pe], term1);
+                                if constexpr ((MODE & MODE_ARC2) != 0) {
+                                    const double term2 = su * ((1.0 / (dt * dt) + x) / st);
+                                    acc2 += term2;
+                                    unsafeAtomicAdd(&a.grad2[pe], term2);
+                                }
+                            };
+                            term(pa, sta, da, dta, xa);
+                            if (two) term(pb, stb, db, dtb, xb);
+                        };
+                        uint32_t e_held = 0;
+                        bool held = false;
                         for_row_words(bits, off[u], off[u + 1], [&](uint32_t wi, uint32_t w) {
                             wg_and(&bits[wi], ~w);
                             while (w) {
                                 const uint32_t e = (wi << 5) + static_cast<uint32_t>(__ffs(w) - 1);
                                 w &= w - 1;
-                                const uint32_t t = tgt[e];
-                                const double st = tree ? 1.0 : ldc(&R[t].sigma);
-                                const double term1 = su * ((1.0 + ldc(&R[t].delta1)) / st);
-                                acc1 += term1;
-                                if constexpr ((MODE & MODE_ARC1) != 0) unsafeAtomicAdd(&a.grad1[perm[e]], term1);
-                                if constexpr ((MODE & MODE_ARC2) != 0) {
-                                    const double dt = T::val(ldc(DP(t)));
-                                    const double term2 = su * ((1.0 / (dt * dt) + ldc(&delta2[t])) / st);
-                                    acc2 += term2;
-                                    unsafeAtomicAdd(&a.grad2[perm[e]], term2);
-                                }
+                                if (held) visit(e_held, e, true);
+                                else e_held = e;
+                                held = !held;
                             }
                         });
+                        if (held) visit(e_held, e_held, false);
                         if (ly > 0) {
                             stc(&R[u].delta1, acc1);
                             if constexpr ((MODE & MODE_ARC2) != 0) stc(&delta2[u], acc2);
