@@ -177,15 +177,17 @@ def run_adjoint(args):
     g = ng.Graph(ctx, hg)
     src = np.arange(hg.n, dtype=np.uint32)
     sample = np.arange(0, hg.n, 6, dtype=np.uint32)
-    g.autotune_shape(sample, mode=1)
-    g.autotune_shape(sample, mode=0)
+    d = args.delta  # > 0: a fixed bucket width (and --block) for every pass instead of the tuned shapes
+    if d <= 0.0:
+        g.autotune_shape(sample, mode=1)
+        g.autotune_shape(sample, mode=0)
     steps = args.steps or 3
     for _ in range(args.warmup):
-        g.adjoint(src)
+        g.adjoint(src, delta=d)
     times = {}
     kms = {}
-    for name, fn in (("adjoint", lambda: g.adjoint(src)), ("forward", lambda: g.sssp(src)),
-                     ("brandes", lambda: g.brandes(src))):
+    for name, fn in (("adjoint", lambda: g.adjoint(src, delta=d)), ("forward", lambda: g.sssp(src, delta=d)),
+                     ("brandes", lambda: g.brandes(src, delta=d))):
         fn()
         ctx.synchronize()
         t0 = time.perf_counter()
