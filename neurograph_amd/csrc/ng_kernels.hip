@@ -846,6 +846,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
         uint32_t guard = 0;
         const uint32_t guard_max = 64u * n + 100000u;
         bool aborted = false;
+        // set when the Brandes tree layers left every record but the source's sigma restored (uniform)
+        bool rec_clean = false;
         auto clampn = [&](uint32_t c) -> uint32_t {
             if (c > n) {
                 if (tid == 0) atomicOr(a.err, ERR_OVERFLOW);
@@ -1257,8 +1259,10 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
             // bit for bit). Otherwise a second pass counts the tight in-degrees for the general layers.
             const bool tree = S.cnt[6] + 1u == nS;
             // Brandes on a tree pushes dependencies to parents: every delta is a subtree size, an integer, so the
-            // atomic sums are exact in any order. The sigma layers record each node's parent and clear the bitmap.
+            // atomic sums are exact in any order. The sigma layers record each node's parent beside its order
+            // position (in the second frontier list, idle after the distances are final) and clear the bitmap.
             const bool push_dep = MODE == MODE_BC && tree;
+            uint32_t* const par = Fb;
             if (!tree) {  // tight in-degrees from the bitmap rows (each tight arc once)
                 for (uint32_t i = tid; i < nS; i += BS) {
                     const uint32_t u = Sl[i];
@@ -1298,8 +1302,10 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                             const uint32_t t3 = k > 3 ? tgt[e3] : 0u;
                             const uint32_t pos = atomicAdd(&S.cnt[4], k);
                             auto put = [&](uint32_t j, uint32_t t) {
-                                if (push_dep) stc(&R[t].spare, u);
-                                if (pos + j < n) order[pos + j] = t;
+                                if (pos + j < n) {
+                                    order[pos + j] = t;
+                                    if (push_dep) par[pos + j] = u;
+                                }
                             };
                             put(0, t0);
                             if (k > 1) put(1, t1);
@@ -1359,13 +1365,21 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
             NG_TICK(t_back);
             // ---- reverse layers: dependencies, per-node and per-arc accumulation, bitmap cleared
             if (push_dep) {
+                // A node's record is written only when its dependency is nonzero (the betweenness sum, and the
+                // dependency cleared for the next source, in one line): a leaf's record is read and left clean,
+                // and the reset below leaves every record alone.
+                rec_clean = le == nS;
                 for (int ly = static_cast<int>(nl) - 1; ly >= 1; --ly) {
                     const uint32_t l0 = layers[ly], l1 = layers[ly + 1];
                     for (uint32_t i = l0 + tid; i < l1; i += BS) {
                         const uint32_t t = order[i];
+                        const uint32_t p = par[i];
                         const double d = ldc(&R[t].delta1);
-                        R[t].bc += d;
-                        if (ly > 1) wg_add(&R[ldc(&R[t].spare)].delta1, 1.0 + d);
+                        if (d != 0.0) {
+                            R[t].bc += d;
+                            if (rec_clean) stc(&R[t].delta1, 0.0);
+                        }
+                        if (ly > 1) wg_add(&R[p].delta1, 1.0 + d);
                     }
                     bsync();
                 }
@@ -1446,10 +1460,13 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
         if (aborted) {
             for (uint32_t v = tid; v < n; v += BS) stc(DP(v), T::kInf);
         }
+        if constexpr (MODE != MODE_SSSP) {
+            if (rec_clean && tid == 0) stc(&R[s].sigma, 0.0);  // on a tree only the source has a sigma
+        }
         for (uint32_t i = tid; i < nS; i += BS) {
             const uint32_t v = Sl[i];
             stc(DP(v), T::kInf);
-            if constexpr (MODE != MODE_SSSP) {
+            if constexpr (MODE != MODE_SSSP) if (!rec_clean) {
                 stc(&R[v].sigma, 0.0);
                 stc(&R[v].delta1, 0.0);
                 if constexpr ((MODE & MODE_ARC2) != 0) stc(&delta2[v], 0.0);
