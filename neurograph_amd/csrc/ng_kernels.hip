@@ -1384,6 +1384,10 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                     bsync();
                 }
             } else {
+                // On a tree each node has one parent: the parent clears a child's dependencies once it has read
+                // them, a node's own dependencies are written only when nonzero, and the reset below leaves the
+                // records alone, so a leaf's lines are read and never dirtied.
+                rec_clean = tree && le == nS;
                 for (int ly = static_cast<int>(nl) - 1; ly >= 0; --ly) {
                     const uint32_t l0 = layers[ly], l1 = layers[ly + 1];
                     for (uint32_t i = l0 + tid; i < l1; i += BS) {
@@ -1410,6 +1414,14 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                                 dtb = T::val(ldc(DP(tb)));
                                 xa = ldc(&delta2[ta]);
                                 xb = ldc(&delta2[tb]);
+                            }
+                            if (rec_clean) {
+                                if (da != 0.0) stc(&R[ta].delta1, 0.0);
+                                if (two && db != 0.0) stc(&R[tb].delta1, 0.0);
+                                if constexpr ((MODE & MODE_ARC2) != 0) {
+                                    if (xa != 0.0) stc(&delta2[ta], 0.0);
+                                    if (two && xb != 0.0) stc(&delta2[tb], 0.0);
+                                }
                             }
                             auto term = [&](uint32_t pe, double st, double dl, double dt, double x) {
                                 const double term1 = su * ((1.0 + dl) / st);
@@ -1438,9 +1450,11 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                         });
                         if (held) visit(e_held, e_held, false);
                         if (ly > 0) {
-                            stc(&R[u].delta1, acc1);
-                            if constexpr ((MODE & MODE_ARC2) != 0) stc(&delta2[u], acc2);
-                            if constexpr ((MODE & MODE_BC) != 0) R[u].bc += acc1;
+                            if (!rec_clean || acc1 != 0.0) {
+                                stc(&R[u].delta1, acc1);
+                                if constexpr ((MODE & MODE_BC) != 0) R[u].bc += acc1;
+                            }
+                            if constexpr ((MODE & MODE_ARC2) != 0) if (!rec_clean || acc2 != 0.0) stc(&delta2[u], acc2);
                         }
                     }
                     bsync();
