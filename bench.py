@@ -1,13 +1,13 @@
 """Benchmark of the MI355X APSP + Brandes path.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload spatial50k|spatial200k|fixed5k]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload spatial200k|spatial50k|fixed5k|cohort64|adjoint20k]
 
 One step is one batched call of the hot path (ng_brandes_accumulate with per-source SSSP summaries, so
 every source yields both its APSP summary and its betweenness contribution) over one batch of sources of
 a synthetic graph resident in HBM. The timed region is exactly K steps, bracketed by device
 synchronisation; value = sources processed / wall time.
 
-Workloads (default spatial50k):
+Workloads (default spatial200k, the headline configuration):
   fixed5k      5000 nodes, exactly 6 distinct out-arcs per node, integer weights U{1..10}, seed 0, all sources
   spatial50k   50k nodes uniform in a 1000^3 um box, p ~ exp(-d/150 um), mean out-degree 20, FP64 lognormal(0,0.5),
                all 50k sources
@@ -22,15 +22,22 @@ Workloads (default spatial50k):
 Roofline: algorithmic bytes per source B_bc = 2*(A*(s_i+s_w) + (V+1)*s_o) + V*(2*s_d + 32) (one pass over the
 CSR for the forward sweep, one for the backward sweep, per-node distance/sigma/delta/betweenness state) divided
 by the batched kernel's launch duration (HIP events on the library's stream), against 8 TB/s.
-CPU baseline: the oracle's restatement of the reference (PriorityQueueVariant APSP + BrandesVariant), one
-worker thread per core over contiguous source blocks, timed on a bounded sample of the same sources.
+Traffic: the hot kernel's profiled bytes per launch (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled by
+what the counter reads for a plain copy of known size), read from the committed profile of the workload; these
+are L2-to-fabric bytes (they include hits in the MALL, the last-level cache in front of HBM).
+CPU baseline: the oracle's restatement of the reference (PriorityQueueVariant APSP + BrandesVariant), one worker
+thread per host core available to this process (its CPU affinity, capped by its cgroup CPU quota and by
+OMP_NUM_THREADS where those are set), over contiguous source blocks, timed on a bounded sample of the same
+sources; the median of 3 timed runs.
 """
 
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
+import subprocess
 import sys
 import time
 
@@ -68,13 +75,79 @@ def bytes_per_source(g) -> float:
 
 
 def read_traffic(workload: str):
-    """HBM bytes per launch of the hot kernel from the committed rocprofv3 counter summary of this workload
-    (FETCH_SIZE + WRITE_SIZE, each scaled by what the counter reads for a copy of known size), or None."""
-    prof = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    """(bytes per launch, source) of the hot kernel from the committed rocprofv3 counter summary of this workload
+    (FETCH_SIZE + WRITE_SIZE, each scaled by what the counter reads for a copy of known size), or (None, None).
+    The counters count L2-to-fabric bytes, MALL hits included."""
+    rel = os.path.join("profiles", f"traffic_{workload}.json")
+    prof = os.path.join(ROOT, rel)
     if not os.path.exists(prof):
-        return None
+        return None, None
     with open(prof) as f:
-        return json.load(f).get("hbm_bytes_per_launch")
+        return json.load(f).get("hbm_bytes_per_launch"), rel
+
+
+def roofline(achieved_gbs: float, workload: str) -> dict:
+    traffic, src = read_traffic(workload)
+    return {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "traffic_kind": "profiled L2-to-fabric bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated; MALL hits "
+                            "included)" if traffic else None,
+            "traffic_source": src}
+
+
+def host_cpu() -> dict:
+    """Host cores this process may use: CPU affinity, cgroup CPU quota (cgroup v2 cpu.max or v1 cfs quota), and
+    OMP_NUM_THREADS where set; threads = the smallest of them. Plus the lscpu model name."""
+    info = {"nproc": os.cpu_count() or 1}
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        info["affinity"] = info["nproc"]
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+            if q != "max":
+                quota = int(q) / int(per)
+    except (OSError, ValueError):
+        try:
+            with open("/sys/fs/cgroup/cpu/cpu.cfs_quota_us") as f:
+                q = int(f.read())
+            with open("/sys/fs/cgroup/cpu/cpu.cfs_period_us") as f:
+                per = int(f.read())
+            if q > 0:
+                quota = q / per
+        except (OSError, ValueError):
+            pass
+    info["cgroup_cpus"] = None if quota is None else round(quota, 2)
+    omp = os.environ.get("OMP_NUM_THREADS")
+    info["omp_num_threads"] = int(omp) if omp and omp.isdigit() else None
+    caps = [info["affinity"]]
+    if quota is not None:
+        caps.append(max(1, math.floor(quota)))
+    if info["omp_num_threads"]:
+        caps.append(info["omp_num_threads"])
+    info["threads"] = max(1, min(caps))
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if model is None:
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        model = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+    info["model"] = model
+    return info
 
 
 def copy_calibration(ctx, mib: int):
@@ -144,7 +217,6 @@ def run_cohort(args):
     plain_value = n_total * steps / plain_wall
     algo = sum(bytes_per_source(h) * h.n for h in hgs)
     achieved = algo * steps / (sum(kernel_ms) / 1e3) / 1e9
-    traffic = read_traffic("cohort64")
     res = {
         "metric": METRIC, "value": round(value, 3), "unit": "sources/s", "n_gpus": 1, "steps": steps,
         "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / steps, 3), "higher_is_better": True, "scaling": "weak",
@@ -159,8 +231,7 @@ def run_cohort(args):
                               "ms_per_step": round(plain_wall * 1e3 / steps, 3),
                               "kernel_ms_per_step": round(plain_kernel / steps, 3)},
         "cohort_vs_plain": round(value / plain_value, 3),
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "roofline": roofline(achieved, "cohort64"),
     }
     if not args.no_cpu:
         print("[bench] timing the CPU baseline", file=sys.stderr, flush=True)
@@ -178,6 +249,8 @@ def run_adjoint(args):
     src = np.arange(hg.n, dtype=np.uint32)
     sample = np.arange(0, hg.n, 6, dtype=np.uint32)
     d = args.delta  # > 0: a fixed bucket width (and --block) for every pass instead of the tuned shapes
+    if args.block:
+        g.set_options(block=args.block)
     if d <= 0.0:
         g.autotune_shape(sample, mode=1)
         g.autotune_shape(sample, mode=0)
@@ -214,8 +287,7 @@ def run_adjoint(args):
         "plain_forward": {"value": round(hg.n / times["forward"], 3), "unit": "sources/s", "kernel_ms": round(kms["forward"], 3)},
         "plain_brandes": {"value": round(hg.n / times["brandes"], 3), "unit": "sources/s", "kernel_ms": round(kms["brandes"], 3)},
         "adjoint_vs_forward": round(times["forward"] / times["adjoint"], 3),
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic("adjoint20k")},
+        "roofline": roofline(achieved, "adjoint20k"),
     }
     print(json.dumps(res), flush=True)
 
@@ -225,7 +297,8 @@ def cpu_baseline_cohort(hgs, seconds: float):
     graph are independent); the rate is the cohort's total sources over the extrapolated total time."""
     from oracle import oracle as orc
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    host = host_cpu()
+    threads = host["threads"]
     total_t, sampled = 0.0, 0
     per_graph = seconds / len(hgs)
     for h in hgs:
@@ -244,35 +317,44 @@ def cpu_baseline_cohort(hgs, seconds: float):
         sampled += len(src)
     n_total = sum(h.n for h in hgs)
     return {"value": round(n_total / total_t, 3), "unit": "sources/s", "cores": threads, "kind": "port",
+            "host": host,
             "sample": f"{sampled} sources spread over all 64 graphs, each graph's time extrapolated to all its sources; "
                       f"avg-apsp + avg-centrality (oracle/oracle.cpp, {threads} threads)"}
 
 
-def cpu_baseline(g, sources, seconds: float):
-    """Oracle restatement of the reference's CPU path (APSP via PriorityQueueVariant + BrandesVariant), all host
-    cores, on a bounded sample of the same sources (sources are independent, so the rate scales linearly)."""
+def cpu_baseline(g, sources, seconds: float, runs: int = 3):
+    """Oracle restatement of the reference's CPU path (APSP via PriorityQueueVariant + BrandesVariant: the
+    reference's avg-apsp and avg-centrality runs), one worker thread per available host core, each a contiguous
+    block of sources like one MPI rank per core, on a bounded sample of the same sources (sources are independent,
+    so the rate scales linearly). `runs` timed runs of about seconds / runs each; the median is reported."""
     from oracle import oracle as orc
 
+    host = host_cpu()
+    threads = host["threads"]
     c = orc.Csr(g.n, g.offsets, g.targets, g.weights)
-    threads = max(1, min(16, os.cpu_count() or 1))
     probe = sources[: 2 * threads]
     t0 = time.perf_counter()
     orc.apsp(c, sources=probe, threads=threads)
     orc.brandes(c, sources=probe, threads=threads)
-    per_batch = time.perf_counter() - t0
-    per_source_wall = per_batch / len(probe)
-    k = int(max(len(probe), min(len(sources), seconds / max(per_source_wall, 1e-9))))
+    per_source_wall = (time.perf_counter() - t0) / len(probe)
+    k = int(max(len(probe), min(len(sources), seconds / runs / max(per_source_wall, 1e-9))))
     k = max(threads, (k // threads) * threads)
     sample = sources[:k]
-    t0 = time.perf_counter()
-    orc.apsp(c, sources=sample, threads=threads)
-    t1 = time.perf_counter()
-    orc.brandes(c, sources=sample, threads=threads)
-    t2 = time.perf_counter()
-    total = t2 - t0
-    return {"value": round(len(sample) / total, 3), "unit": "sources/s", "cores": threads, "kind": "port",
-            "sample": f"{len(sample)} of the workload's sources; avg-apsp {t1 - t0:.2f} s + avg-centrality {t2 - t1:.2f} s "
-                      f"(oracle/oracle.cpp, {threads} threads, contiguous source blocks)"}
+    rates, parts = [], []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        orc.apsp(c, sources=sample, threads=threads)
+        t1 = time.perf_counter()
+        orc.brandes(c, sources=sample, threads=threads)
+        t2 = time.perf_counter()
+        rates.append(len(sample) / (t2 - t0))
+        parts.append((t1 - t0, t2 - t1))
+        print(f"[bench] cpu run: {len(sample)} sources in {t2 - t0:.2f} s", file=sys.stderr, flush=True)
+    med = sorted(range(runs), key=lambda i: rates[i])[runs // 2]
+    return {"value": round(rates[med], 3), "unit": "sources/s", "cores": threads, "kind": "port",
+            "runs": [round(r, 3) for r in rates], "statistic": f"median of {runs}", "host": host,
+            "sample": f"{len(sample)} of the workload's sources per run; median run avg-apsp {parts[med][0]:.2f} s + "
+                      f"avg-centrality {parts[med][1]:.2f} s (oracle/oracle.cpp, {threads} threads, contiguous source blocks)"}
 
 
 def main():
@@ -280,10 +362,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0, help="timed steps (default: one pass over the workload's sources)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="spatial50k",
+    ap.add_argument("--workload", default="spatial200k",
                     choices=["fixed5k", "spatial50k", "spatial200k", "cohort64", "adjoint20k"])
     ap.add_argument("--batch", type=int, default=0)
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-seconds", type=float, default=24.0, help="CPU baseline budget, split over 3 timed runs")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--delta", type=float, default=0.0, help="bucket width (0: autotune width and lanes per source)")
     ap.add_argument("--block", type=int, default=0, choices=[0, 64, 256, 512, 1024],
@@ -296,8 +378,6 @@ def main():
               "one final all-reduce of the n-entry betweenness vector and of the scalar sums)", file=sys.stderr)
         sys.exit(2)
 
-    if args.block:
-        os.environ["NG_BLOCK_SIZE"] = str(args.block)  # read by the library for graphs without a tuned shape
     import neurograph_amd as ng
 
     if args.workload == "cohort64":
@@ -314,6 +394,8 @@ def main():
     steps = args.steps or len(batches)
     ctx = ng.Context(0)
     g = ng.Graph(ctx, hg)
+    if args.block:
+        g.set_options(block=args.block)  # lanes per workgroup for every call (with --delta: no autotuning)
     print(f"[bench] {args.workload}: n={hg.n} m={hg.m} generated in {gen_s:.1f}s; {len(sources)} sources, batch {batch}, "
           f"{steps} steps", file=sys.stderr, flush=True)
 
@@ -353,7 +435,6 @@ def main():
     copy_calibration(ctx, args.copy_calibration_mib)
     launches_bytes = [bytes_per_source(hg) * len(batches[k % len(batches)]) for k in range(steps)]
     achieved = sum(launches_bytes) / (sum(kernel_ms) / 1e3) / 1e9
-    traffic = read_traffic(args.workload)
     res = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -373,8 +454,7 @@ def main():
                    "outputs": "per-source APSP summaries + betweenness in one pass"},
         "gteps": round(arcs / wall / 1e9, 3),
         "kernel_ms_per_step": round(sum(kernel_ms) / steps, 3),
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "roofline": roofline(achieved, args.workload),
     }
     if not args.no_cpu:
         print("[bench] timing the CPU baseline", file=sys.stderr, flush=True)

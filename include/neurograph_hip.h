@@ -111,6 +111,33 @@ void ng_graph_destroy(ng_graph* g);
 /* Distance element type of dist_out for this graph and length kind. */
 ng_status ng_graph_dist_kind(const ng_graph* g, ng_length_kind lkind, ng_dist_kind* out);
 
+/* Launch options of one graph: explicit overrides of the library's automatic launch shape and capacities, for
+ * tuning sweeps and for tests that must reach a particular code path. No reference counterpart (the reference
+ * has no launch shape). Every option defaults to automatic; results never depend on an option, only speed and
+ * which code path runs. The library reads no environment variables.
+ *   NG_OPT_BLOCK          lanes per source workgroup for every call on this graph: 64, 256, 512 or 1024;
+ *                         0 = the tuned shape (ng_autotune) or the default (256). The autotuner's own trial
+ *                         runs are not affected.
+ *   NG_OPT_MAX_SLOTS      at most this many concurrent source slots (workgroups); 0 = as many as are resident.
+ *   NG_OPT_LDS_MAPS       -1 = automatic; 0..31 = the extra LDS maps for every call (bit flags: 1 settle passes
+ *                         in increasing node id, 2 frontier bitmap, 4 16-bit distance bounds, 8 4-bit distance
+ *                         codes, 16 the distances themselves in LDS; maps that do not fit are dropped).
+ *   NG_OPT_PILE_CAPACITY  parked-node entries per pile list; 0 = 3n + 1024, which is also the largest value
+ *                         used; smaller values are raised to n + 2048 (a full pile makes bucket advances reload
+ *                         distances: slower, same results).
+ *   NG_OPT_LOG_CAPACITY   candidate tight-arc log entries per slot (betweenness modes); 0 = min(m, max(8n, 4096));
+ *                         a source whose candidates overflow it rescans all rows of its reached nodes instead
+ *                         (ng_call_stats.flags bit 2). */
+typedef enum {
+    NG_OPT_BLOCK = 0,
+    NG_OPT_MAX_SLOTS = 1,
+    NG_OPT_LDS_MAPS = 2,
+    NG_OPT_PILE_CAPACITY = 3,
+    NG_OPT_LOG_CAPACITY = 4
+} ng_graph_option;
+ng_status ng_graph_set_option(ng_graph* g, ng_graph_option opt, int64_t value);
+ng_status ng_graph_get_option(const ng_graph* g, ng_graph_option opt, int64_t* value);
+
 /* Multi-source SSSP with per-source summaries, one batched launch for all sources. Replaces the
  * per-source loop of AllPairsShortestPath::compute_apsp_impl (AllPairsShortestPath.h:130-143) over
  * Dijkstra::compute_sssp (Dijkstra.h:56-98) + summarize_distances (SsspResult.h:43-64), and the

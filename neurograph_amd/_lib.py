@@ -19,6 +19,9 @@ NG_OK, NG_EINVAL, NG_ENOMEM, NG_EDEVICE, NG_EUNSUPPORTED = 0, 1, 2, 3, 4
 NG_W_I32, NG_W_F32, NG_W_F64 = 0, 1, 2
 NG_LEN_ABS, NG_LEN_INVERSE, NG_LEN_RAW = 0, 1, 2
 NG_DIST_U64, NG_DIST_F32, NG_DIST_F64 = 0, 1, 2
+NG_OPT_BLOCK, NG_OPT_MAX_SLOTS, NG_OPT_LDS_MAPS, NG_OPT_PILE_CAPACITY, NG_OPT_LOG_CAPACITY = 0, 1, 2, 3, 4
+_OPTIONS = {"block": NG_OPT_BLOCK, "max_slots": NG_OPT_MAX_SLOTS, "lds_maps": NG_OPT_LDS_MAPS,
+            "pile_capacity": NG_OPT_PILE_CAPACITY, "log_capacity": NG_OPT_LOG_CAPACITY}
 
 _WKIND = {np.dtype(np.int32): NG_W_I32, np.dtype(np.float32): NG_W_F32, np.dtype(np.float64): NG_W_F64}
 _WDTYPE = {NG_W_I32: np.int32, NG_W_F32: np.float32, NG_W_F64: np.float64}
@@ -77,6 +80,8 @@ def lib() -> ctypes.CDLL:
         "ng_graph_create": (i32, [vp, u32, u64, vp, vp, vp, i32, P(vp)]),
         "ng_graph_destroy": (None, [vp]),
         "ng_graph_dist_kind": (i32, [vp, i32, P(i32)]),
+        "ng_graph_set_option": (i32, [vp, i32, ctypes.c_int64]),
+        "ng_graph_get_option": (i32, [vp, i32, P(ctypes.c_int64)]),
         "ng_sssp_summaries": (i32, [vp, vp, u32, i32, dbl, vp, vp]),
         "ng_brandes_accumulate": (i32, [vp, vp, u32, dbl, vp, vp]),
         "ng_distance_histogram": (i32, [vp, vp, u32, i32, dbl, dbl, u32, vp, vp]),
@@ -260,6 +265,17 @@ class Graph:
         if sources is None:
             return np.arange(self.n, dtype=np.uint32)
         return np.ascontiguousarray(sources, dtype=np.uint32)
+
+    def set_options(self, **opts) -> "Graph":
+        """Launch options (ng_graph_set_option): block, max_slots, lds_maps, pile_capacity, log_capacity."""
+        for k, v in opts.items():
+            _check(lib().ng_graph_set_option(self.h, _OPTIONS[k], int(v)))
+        return self
+
+    def get_option(self, name: str) -> int:
+        v = ctypes.c_int64()
+        _check(lib().ng_graph_get_option(self.h, _OPTIONS[name], ctypes.byref(v)))
+        return v.value
 
     def dist_dtype(self, lkind: int = NG_LEN_ABS):
         k = ctypes.c_int()

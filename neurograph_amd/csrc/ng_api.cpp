@@ -99,6 +99,12 @@ struct ng_graph {
     bool has_negative = false;
     View views[2];                 // [0] |w|, [1] 1/|w|
     ng_call_stats last{};
+    // launch options (ng_graph_set_option): 0 / -1 = automatic
+    int opt_block = 0;
+    uint32_t opt_max_slots = 0;
+    int opt_lds_maps = -1;
+    uint32_t opt_pile_capacity = 0;
+    uint32_t opt_log_capacity = 0;
 };
 
 // Cohort: all sources of many graphs in one persistent launch, captured once in a hipGraph and replayed.
@@ -308,10 +314,15 @@ void free_ws(Workspace& w) {
 // bounds first: MAP_BOUND skips the distance probes of candidates above their target's bound (74 % of the
 // probes on a 50k-node spatial graph), with 4-bit codes (MAP_BOUND4, when allowed) where 16-bit bounds do
 // not fit; MAP_SORTED makes settle passes expand each bucket's settled nodes in increasing id; MAP_FRONTIER
-// replaces the global frontier stamps. NG_LDS_MAPS=0..15 overrides the choice (tuning experiments).
-int lds_maps_for(DKind dk, int mode, uint32_t n, int block, bool allow4) {
+// replaces the global frontier stamps. forced >= 0 (the graph's NG_OPT_LDS_MAPS) overrides the choice; LDS
+// distances that do not fit are dropped from it.
+int lds_maps_for(DKind dk, int mode, uint32_t n, int block, bool allow4, int forced = -1) {
     const int b = allow4 ? (MAP_BOUND | MAP_BOUND4) : static_cast<int>(MAP_BOUND);
-    if (const char* e = std::getenv("NG_LDS_MAPS")) return std::atoi(e) & (allow4 ? 31 : 23);
+    if (forced >= 0) {
+        int maps = forced & (allow4 ? 31 : 23);
+        if ((maps & MAP_LDIST) != 0 && (lds_plan_maps(dk, n, maps) & MAP_LDIST) == 0) maps &= ~MAP_LDIST;
+        return maps;
+    }
     const int base = batch_blocks_per_cu(dk, mode, n, 0, block);
     // graphs small enough: the distances themselves in LDS (256- and 1024-lane workgroups)
     if (block == 256 || block == 1024)
@@ -327,14 +338,14 @@ int lds_maps_for(DKind dk, int mode, uint32_t n, int block, bool allow4) {
     return 0;
 }
 
-// Lanes per workgroup for a call: explicit, else tuned, else NG_BLOCK_SIZE (experiments), else the default.
-int pick_block(const View& v, int mode_idx, int requested) {
-    if (requested == 64 || requested == 256 || requested == 512 || requested == 1024) return requested;
+bool valid_block(int b) { return b == 64 || b == 256 || b == 512 || b == 1024; }
+
+// Lanes per workgroup for a call: requested by the caller (the autotuner's trials), else the graph's
+// NG_OPT_BLOCK, else tuned, else the default.
+int pick_block(const View& v, int mode_idx, int requested, int option) {
+    if (valid_block(requested)) return requested;
+    if (valid_block(option)) return option;
     if (v.tuned_block[mode_idx] > 0) return v.tuned_block[mode_idx];
-    if (const char* e = std::getenv("NG_BLOCK_SIZE")) {
-        const int b = std::atoi(e);
-        if (b == 64 || b == 256 || b == 512 || b == 1024) return b;
-    }
     return kBlock;
 }
 
@@ -353,31 +364,38 @@ uint64_t slot_bytes(uint32_t n, uint64_t m, int mode, uint32_t logcap) {
     return b;
 }
 
-uint32_t log_capacity(uint32_t n, uint64_t m) {
+// Candidate tight-arc log entries per slot; `option` > 0 (NG_OPT_LOG_CAPACITY) replaces the default.
+uint32_t log_capacity(uint32_t n, uint64_t m, uint32_t option = 0) {
+    if (option > 0) return option;
     const uint64_t want = std::max<uint64_t>(static_cast<uint64_t>(n) * 8, 4096);
     return static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>(m, 1), want));
 }
 
-// Pile entries per list for a slot stride of n nodes; NG_PILE_CAPACITY shrinks it (tests use it to force the
-// exact bucket-advance path).
-uint32_t wanted_pcap(uint32_t n) {
+// Pile entries per list for a slot stride of n nodes; `option` > 0 (NG_OPT_PILE_CAPACITY) shrinks it, down to
+// n + 2 x the largest workgroup (tests use it to force the exact bucket-advance path).
+uint32_t wanted_pcap(uint32_t n, uint32_t option = 0) {
     uint32_t p = pile_capacity(n);
-    if (const char* pc = std::getenv("NG_PILE_CAPACITY")) {
-        const long long c = std::atoll(pc);
-        if (c > 0) p = static_cast<uint32_t>(std::max<long long>(c, static_cast<long long>(n) + 2 * kMaxBlock));
-    }
+    if (option > 0) p = static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>(option, static_cast<uint64_t>(n) + 2 * kMaxBlock), p));
     return p;
 }
+
+// Capacities a call asks of the workspace (the graph's options; the cohort takes the defaults).
+struct WsCaps {
+    uint32_t pile = 0;
+    uint32_t log = 0;
+};
 
 // Makes sure workspace w holds at least `slots` slots for graphs of up to n nodes and m arcs, with every
 // array `mode` needs and dist in kind dk. Growing keeps the larger of the old and new dimensions, except that
 // old slots beyond `slot_cap` (the caller's memory budget at the new dimensions) are dropped: many small
 // slots of a small graph must not be carried over to a large one.
 ng_status ensure_ws(Workspace& w, hipStream_t st, uint32_t n, uint64_t m, uint32_t slots, int mode, DKind dk, uint32_t n_sources,
-                    uint32_t slot_cap) {
+                    uint32_t slot_cap, WsCaps caps = {}) {
     int need_bits = (mode == MODE_SSSP) ? 0 : (MODE_BC | MODE_ARC1 | MODE_ARC2);
+    const bool has_log = ((need_bits | w.mode_bits) & MODE_BC) != 0;
     const bool realloc = w.slots < slots || w.n < n || w.m < m || dk_size(dk) > w.dist_bytes || (need_bits & ~w.mode_bits) != 0 ||
-                         w.pcap != wanted_pcap(std::max(w.n, n));
+                         w.pcap != wanted_pcap(std::max(w.n, n), caps.pile) ||
+                         (has_log && w.logcap != log_capacity(std::max(w.n, n), std::max(w.m, m), caps.log));
     if (realloc) {
         if (w.dist != nullptr) {
             n = std::max(n, w.n);
@@ -397,14 +415,14 @@ ng_status ensure_ws(Workspace& w, hipStream_t st, uint32_t n, uint64_t m, uint32
         NG_HIP(hipMalloc(&w.fstamp, 4 * std::max<size_t>(ns, 1)));
         NG_HIP(hipMalloc(&w.sstamp, 4 * std::max<size_t>(ns, 1)));
         NG_HIP(hipMalloc(&w.lists, 12 * std::max<size_t>(ns, 1)));
-        w.pcap = wanted_pcap(n);
+        w.pcap = wanted_pcap(n, caps.pile);
         NG_HIP(hipMalloc(&w.pile, 16ull * 2 * w.pcap * slots));
         NG_HIP(hipMalloc(&w.counters, 16 * static_cast<size_t>(slots)));
         NG_HIP(launch_fill_u32(w.fstamp, ns, 0u, st));
         NG_HIP(launch_fill_u32(w.sstamp, ns, 0u, st));
         NG_HIP(launch_fill_u32(w.counters, 4ull * slots, 1u, st));
         if (need_bits) {
-            w.logcap = log_capacity(n, m);
+            w.logcap = log_capacity(n, m, caps.log);
             w.words = (m + 31) / 32;
             NG_HIP(hipMalloc(&w.rec, sizeof(BcRec) * std::max<size_t>(ns, 1)));
             NG_HIP(hipMalloc(&w.delta2, 8 * std::max<size_t>(ns, 1)));
@@ -489,33 +507,31 @@ ng_status run_batch(ng_graph* g, const RunSpec& rs, uint32_t* slots_used, double
         v.lend_delta = delta;
     }
     // slots: resident workgroups of this kernel, bounded by the sources and by device memory
-    const int block = pick_block(v, mode_idx, rs.block);
-    const int maps = lds_maps_for(v.dk, rs.mode, g->n, block, !rs.no_bound4);
+    const int block = pick_block(v, mode_idx, rs.block, g->opt_block);
+    const int maps = lds_maps_for(v.dk, rs.mode, g->n, block, !rs.no_bound4, g->opt_lds_maps);
     if (lds_plan_bound_kind(v.dk, g->n, maps) == 2 && !(v.bound_inv_q > 0.0)) {
         st = estimate_bound_scale(g, rs.vi, delta);
         if (st != NG_OK) return st;
     }
     const int bpc = batch_blocks_per_cu(v.dk, rs.mode, g->n, maps, block);
     uint64_t slots = static_cast<uint64_t>(bpc) * static_cast<uint64_t>(ctx->cus);
-    // NG_MAX_SLOTS caps the concurrent source slots (for tuning sweeps and experiments)
-    if (const char* cap = std::getenv("NG_MAX_SLOTS")) {
-        const long long c = std::atoll(cap);
-        if (c > 0) slots = std::min<uint64_t>(slots, static_cast<uint64_t>(c));
-    }
+    // NG_OPT_MAX_SLOTS caps the concurrent source slots (tuning sweeps, tests of slot reuse)
+    if (g->opt_max_slots > 0) slots = std::min<uint64_t>(slots, g->opt_max_slots);
     slots = std::min<uint64_t>(slots, std::max<uint32_t>(rs.n_sources, 1));
     size_t free_b = 0, total_b = 0;
     NG_HIP(hipMemGetInfo(&free_b, &total_b));
     Workspace& w = ctx->ws;
     const uint32_t wn = std::max(w.n, g->n);
     const uint64_t wm = std::max(w.m, g->m);
-    const uint64_t per = slot_bytes(wn, wm, rs.mode, log_capacity(wn, wm));
+    const WsCaps caps{g->opt_pile_capacity, g->opt_log_capacity};
+    const uint64_t per = slot_bytes(wn, wm, rs.mode, log_capacity(wn, wm, caps.log));
     // memory already held by the context's workspace counts as available
-    const uint64_t held = static_cast<uint64_t>(w.slots) * slot_bytes(w.n, w.m, rs.mode, log_capacity(w.n, w.m));
+    const uint64_t held = static_cast<uint64_t>(w.slots) * slot_bytes(w.n, w.m, rs.mode, w.logcap);
     const uint64_t budget = static_cast<uint64_t>(static_cast<double>(free_b + held) * 0.6);
     const uint64_t slot_cap = per > 0 ? std::max<uint64_t>(1, budget / per) : slots;
     slots = std::min<uint64_t>(slots, slot_cap);
     st = ensure_ws(w, ctx->stream, g->n, g->m, static_cast<uint32_t>(slots), rs.mode, v.dk, rs.n_sources,
-                   static_cast<uint32_t>(std::min<uint64_t>(slot_cap, 0xFFFFFFFFu)));
+                   static_cast<uint32_t>(std::min<uint64_t>(slot_cap, 0xFFFFFFFFu)), caps);
     if (st != NG_OK) return st;
     const hipStream_t s = ctx->stream;
     if (rs.n_sources) NG_HIP(hipMemcpyAsync(w.sources, rs.sources, sizeof(uint32_t) * rs.n_sources, hipMemcpyHostToDevice, s));
@@ -778,6 +794,46 @@ ng_status ng_graph_dist_kind(const ng_graph* g, ng_length_kind lkind, ng_dist_ki
     if (st != NG_OK) return st;
     *out = static_cast<ng_dist_kind>(dist_kind_of(g->views[0].dk));
     return NG_OK;
+}
+
+ng_status ng_graph_set_option(ng_graph* g, ng_graph_option opt, int64_t value) {
+    if (check_graph(g) != NG_OK) return NG_EINVAL;
+    switch (opt) {
+    case NG_OPT_BLOCK:
+        if (value != 0 && !valid_block(static_cast<int>(value))) return fail(NG_EINVAL, "lanes per workgroup must be 0, 64, 256, 512 or 1024, not %lld", (long long)value);
+        g->opt_block = static_cast<int>(value);
+        return NG_OK;
+    case NG_OPT_MAX_SLOTS:
+        if (value < 0 || value > 0xFFFFFFFFll) return fail(NG_EINVAL, "slot cap %lld out of range", (long long)value);
+        g->opt_max_slots = static_cast<uint32_t>(value);
+        return NG_OK;
+    case NG_OPT_LDS_MAPS:
+        if (value < -1 || value > 31) return fail(NG_EINVAL, "LDS maps must be -1 (automatic) or 0..31, not %lld", (long long)value);
+        g->opt_lds_maps = static_cast<int>(value);
+        return NG_OK;
+    case NG_OPT_PILE_CAPACITY:
+        if (value < 0 || value > 0xFFFFFF00ll) return fail(NG_EINVAL, "pile capacity %lld out of range", (long long)value);
+        g->opt_pile_capacity = static_cast<uint32_t>(value);
+        return NG_OK;
+    case NG_OPT_LOG_CAPACITY:
+        if (value < 0 || value > 0xFFFFFFFFll) return fail(NG_EINVAL, "log capacity %lld out of range", (long long)value);
+        g->opt_log_capacity = static_cast<uint32_t>(value);
+        return NG_OK;
+    }
+    return fail(NG_EINVAL, "unknown graph option %d", static_cast<int>(opt));
+}
+
+ng_status ng_graph_get_option(const ng_graph* g, ng_graph_option opt, int64_t* value) {
+    if (check_graph(g) != NG_OK) return NG_EINVAL;
+    if (!value) return fail(NG_EINVAL, "null output pointer");
+    switch (opt) {
+    case NG_OPT_BLOCK: *value = g->opt_block; return NG_OK;
+    case NG_OPT_MAX_SLOTS: *value = g->opt_max_slots; return NG_OK;
+    case NG_OPT_LDS_MAPS: *value = g->opt_lds_maps; return NG_OK;
+    case NG_OPT_PILE_CAPACITY: *value = g->opt_pile_capacity; return NG_OK;
+    case NG_OPT_LOG_CAPACITY: *value = g->opt_log_capacity; return NG_OK;
+    }
+    return fail(NG_EINVAL, "unknown graph option %d", static_cast<int>(opt));
 }
 
 ng_status ng_sssp_summaries(ng_graph* g, const uint32_t* sources, uint32_t n_sources, ng_length_kind lkind, double delta,
@@ -1108,7 +1164,7 @@ ng_status cohort_build(ng_cohort* c) {
     const uint32_t wn = std::max(w.n, c->n_max);
     const uint64_t wm = std::max(w.m, c->m_max);
     const uint64_t per = slot_bytes(wn, wm, MODE_BC, log_capacity(wn, wm)) + 16ull * wn;
-    const uint64_t held = static_cast<uint64_t>(w.slots) * slot_bytes(w.n, w.m, MODE_BC, log_capacity(w.n, w.m));
+    const uint64_t held = static_cast<uint64_t>(w.slots) * slot_bytes(w.n, w.m, MODE_BC, w.logcap);
     const uint64_t budget = static_cast<uint64_t>(static_cast<double>(free_b + held) * 0.5);
     slots = std::min<uint64_t>(slots, std::max<uint64_t>(1, budget / per));
     // flush layout: slot b touches graph g when one of its items b + kS falls into g's item range

@@ -19,6 +19,20 @@ constexpr int kMaxBlock = 1024;
 // fill wider workgroups, while larger ones would hold thousands of slots of per-node state.
 constexpr uint32_t kWaveSourceMaxNodes = 32768;
 
+// Iteration guard of one source in the batched kernel: at most 64 n + 100 000 frontier phases and bucket
+// advances, computed in 64 bits and saturated to 2^32 - 1 (the guard counter is 32-bit), so that the bound
+// grows with n instead of wrapping for graphs of 2^26 nodes and more.
+__host__ __device__ constexpr uint32_t guard_limit(uint32_t n) {
+    const uint64_t g = 64ull * n + 100000ull;
+    return g > 0xFFFFFFFFull ? 0xFFFFFFFFu : static_cast<uint32_t>(g);
+}
+static_assert(guard_limit(0) == 100000u, "guard of an empty graph");
+static_assert(guard_limit(5000) == 420000u, "guard of a 5000-node graph");
+static_assert(guard_limit(67107301u) == 4294967264u, "largest n whose bound fits 32 bits");
+static_assert(guard_limit(67107302u) == 0xFFFFFFFFu, "saturates where 32-bit arithmetic would wrap");
+static_assert(guard_limit(1u << 26) == 0xFFFFFFFFu, "2^26 nodes: saturated, not wrapped to 100000");
+static_assert(guard_limit(0xFFFFFFFEu) == 0xFFFFFFFFu, "saturates at 2^32 - 1");
+
 // Device distance element type: lengths and distances are kept in one of these.
 enum class DKind : int { U32 = 0, U64 = 1, F32 = 2, F64 = 3 };
 

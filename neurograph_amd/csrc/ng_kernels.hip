@@ -844,7 +844,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
         // Uniform iteration guard: phases and bucket advances are bounded, so a runaway loop (a bug, or a
         // pathological input) ends with ERR_GUARD instead of occupying the device.
         uint32_t guard = 0;
-        const uint32_t guard_max = 64u * n + 100000u;
+        const uint32_t guard_max = guard_limit(n);
         bool aborted = false;
         // set when the Brandes tree layers left every record but the source's sigma restored (uniform)
         bool rec_clean = false;

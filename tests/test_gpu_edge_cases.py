@@ -18,14 +18,14 @@ def ctx():
     return ng.Context(0)
 
 
-def dev(ctx, c):
+def dev(ctx, c, **opts):
     import neurograph_amd as ng
 
-    return ng.Graph(ctx, ng.HostGraph(c.n, c.off, c.tgt, c.w))
+    return ng.Graph(ctx, ng.HostGraph(c.n, c.off, c.tgt, c.w)).set_options(**opts)
 
 
-def check_all(ctx, c, sources, rtol=1e-9):
-    g = dev(ctx, c)
+def check_all(ctx, c, sources, rtol=1e-9, **opts):
+    g = dev(ctx, c, **opts)
     ref = np.stack([orc.sssp(c, int(s))[0] for s in sources]) if len(sources) else None
     summ, dist = g.sssp(sources, want_dist=True)
     if len(sources):
@@ -75,11 +75,10 @@ def test_repeated_sources_accumulate(ctx):
 
 
 @pytest.mark.parametrize("block", ["256", "1024"])
-def test_long_directed_path(ctx, monkeypatch, block):
-    monkeypatch.setenv("NG_BLOCK_SIZE", block)
+def test_long_directed_path(ctx, block):
     n = 20000
     c = orc.Csr.from_edges(n, np.arange(n - 1), np.arange(1, n), np.ones(n - 1, np.int32))
-    g = dev(ctx, c)
+    g = dev(ctx, c, block=int(block))
     sources = np.array([0, 1, 5000, 19998, 19999], np.uint32)
     s = g.sssp(sources, delta=1)
     k = (n - 1 - sources).astype(np.uint64)
@@ -93,9 +92,8 @@ def test_long_directed_path(ctx, monkeypatch, block):
 
 
 @pytest.mark.parametrize("block", ["256", "1024"])
-def test_hub_degree_skew(ctx, monkeypatch, block):
+def test_hub_degree_skew(ctx, block):
     """A 60 000-arc hub row: expansion tiles whose arc count outgrows the chunk owner table's 64-arc chunks."""
-    monkeypatch.setenv("NG_BLOCK_SIZE", block)
     n = 60001
     leaves = np.arange(1, n)
     rng = np.random.default_rng(8)
@@ -103,20 +101,20 @@ def test_hub_degree_skew(ctx, monkeypatch, block):
     dst = np.concatenate([leaves, np.zeros(n - 1, np.int64), rng.integers(1, n, 30000)])
     w = rng.integers(1, 20, len(src)).astype(np.int32)
     c = orc.Csr.from_edges(n, src, dst, w)
-    check_all(ctx, c, [0, 1, 17, 60000])
+    g, _ = check_all(ctx, c, [0, 1, 17, 60000], block=int(block))
+    assert g.last_stats()["block"] == int(block)
 
 
-def test_graph_beyond_lds_maps(ctx, monkeypatch):
+def test_graph_beyond_lds_maps(ctx):
     """800 000 nodes: no LDS bitmap or bound fits, so the settled and frontier stamps live in global memory and
     the wide workgroups' filter stage passes every arc to the slot-by-slot form."""
-    monkeypatch.setenv("NG_BLOCK_SIZE", "1024")
     n, m = 800_000, 2_400_000
     rng = np.random.default_rng(12)
     src = rng.integers(0, n, m)
     dst = (src + rng.integers(1, 2000, m)) % n  # local arcs: long shortest paths, many buckets
     w = rng.integers(1, 50, m).astype(np.int32)
     c = orc.Csr.from_edges(n, src, dst, w)
-    g, _ = check_all(ctx, c, [0, 123_457, 799_999])
+    g, _ = check_all(ctx, c, [0, 123_457, 799_999], block=1024)
     assert g.last_stats()["block"] == 1024
 
 

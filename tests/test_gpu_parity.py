@@ -26,10 +26,12 @@ def ctx():
     return ng.Context(0)
 
 
-def dev_graph(ctx, c):
+def dev_graph(ctx, c, **opts):
+    """Device graph of an oracle CSR; opts are launch options (block, max_slots, lds_maps, pile_capacity,
+    log_capacity) set through ng_graph_set_option."""
     import neurograph_amd as ng
 
-    return ng.Graph(ctx, ng.HostGraph(c.n, c.off, c.tgt, c.w))
+    return ng.Graph(ctx, ng.HostGraph(c.n, c.off, c.tgt, c.w)).set_options(**opts)
 
 
 def rand_graph(n, m, seed, wdtype=np.int32, wlo=1, whi=10, parallel=True, self_loops=True):
@@ -231,15 +233,16 @@ def test_fixed_degree_config_parity(ctx):
     assert agg.average_efficiency == pytest.approx(ref["avg_eff"], rel=1e-12)
 
 
-def test_small_pile_forces_exact_bucket_advance(ctx, monkeypatch):
+def test_small_pile_forces_exact_bucket_advance(ctx):
     """With the pile reserve exhausted, bucket advances reload distances instead of trusting parked values."""
     import neurograph_amd as ng
 
-    monkeypatch.setenv("NG_PILE_CAPACITY", "1")  # clamped to n + 512 entries: re-parking is refused almost always
     own = ng.Context(0)  # a fresh context, so that the workspace stride is this graph's n
     for wdtype in (np.int32, np.float32):
         c = rand_graph(3000, 30000, 77, wdtype=wdtype, wlo=1, whi=20)
-        g = dev_graph(own, c)
+        # clamped to n + 2048 entries: re-parking is refused almost always
+        g = dev_graph(own, c, pile_capacity=1)
+        assert g.get_option("pile_capacity") == 1
         srcs = np.arange(0, 3000, 37)
         ref = np.stack([orc.sssp(c, int(s))[0] for s in srcs])
         for delta in (0.0, 0.5, 3.0):
@@ -251,20 +254,19 @@ def test_small_pile_forces_exact_bucket_advance(ctx, monkeypatch):
 
 @pytest.mark.parametrize("block", ["64", "256", "1024"])
 @pytest.mark.parametrize("maps", [str(k) for k in range(20)])
-def test_lds_bitmap_variants(ctx, monkeypatch, maps, block):
+def test_lds_bitmap_variants(ctx, maps, block):
     """Every combination of the optional LDS maps (settle passes in discovery order or in increasing id;
     frontier de-duplication by global stamps or by an LDS bitmap; probes with no bounds, 16-bit bounds or, for
     maps 8-15, the 4-bit distance codes; for maps 16-19 in 256-lane workgroups, the distances themselves in LDS)
     gives the oracle's distances bit-exactly and its betweenness within
     1e-9, for integer, FP32 and FP64 weights and both length kinds (|w| and 1/|w|), in 64- and 256-lane
     workgroups (plain arc loop) and 1024-lane ones (filter stage, survivors compacted by lane permutes)."""
-    monkeypatch.setenv("NG_LDS_MAPS", maps)
-    monkeypatch.setenv("NG_BLOCK_SIZE", block)
+    opts = {"lds_maps": int(maps), "block": int(block)}
     # int32 weights up to 2^31 - 1 take the 64-bit integer distances, whose bounds round
     for wdtype, seed, wlo, whi in ((np.int32, 5, 1, 9), (np.int32, 9, 1 << 20, 2**31 - 1), (np.float32, 7, 1, 9),
                                    (np.float64, 6, 1, 9)):
         c = rand_graph(4000, 40000, seed, wdtype=wdtype, wlo=wlo, whi=whi)
-        g = dev_graph(ctx, c)
+        g = dev_graph(ctx, c, **opts)
         srcs = np.arange(0, 4000, 29)
         for lkind in (orc.L_ABS, orc.L_INVERSE):
             ref = np.stack([orc.sssp(c, int(s), lkind)[0] for s in srcs])
@@ -276,9 +278,10 @@ def test_lds_bitmap_variants(ctx, monkeypatch, maps, block):
         np.testing.assert_allclose(bc, orc.brandes(c, sources=srcs)["bc"], rtol=1e-9, atol=1e-9)
     # a graph of unit weights ties everywhere (sigma > 1, the general dependency layers)
     c = rand_graph(3000, 30000, 8, wdtype=np.int32, wlo=1, whi=2)
-    g = dev_graph(ctx, c)
+    g = dev_graph(ctx, c, **opts)
     srcs = np.arange(0, 3000, 37)
     np.testing.assert_allclose(g.brandes(srcs), orc.brandes(c, sources=srcs)["bc"], rtol=1e-9, atol=1e-9)
+    assert g.last_stats()["block"] == int(block)
 
 
 def test_shared_workspace_across_graphs(ctx):
@@ -326,16 +329,15 @@ def test_distance_histogram_and_matrix(ctx, wdtype):
 
 
 @pytest.mark.parametrize("block", ["64", "256", "512", "1024"])
-def test_workgroup_sizes(ctx, monkeypatch, block):
+def test_workgroup_sizes(ctx, block):
     """Every workgroup size the kernel is built for (one source per wavefront, or per workgroup of 256, 512 or
     1024 lanes) gives
     the oracle's distances bit-exactly and its betweenness and edge path counts within 1e-9."""
     import neurograph_amd as ng
 
-    monkeypatch.setenv("NG_BLOCK_SIZE", block)
     for wdtype, seed in ((np.int32, 15), (np.float32, 16), (np.float64, 17)):
         c = rand_graph(3000, 30000, seed, wdtype=wdtype, wlo=1, whi=9)
-        g = dev_graph(ctx, c)
+        g = dev_graph(ctx, c, block=int(block))
         srcs = np.arange(0, 3000, 23)
         ref = np.stack([orc.sssp(c, int(s))[0] for s in srcs])
         _, dist = g.sssp(srcs, delta=0.8, want_dist=True)
@@ -359,7 +361,7 @@ def test_autotune_shape(ctx):
     assert g.last_stats()["block"] == b and g.last_stats()["delta"] == d
 
 
-def test_random_configurations(ctx, monkeypatch):
+def test_random_configurations(ctx):
     """30 seeded random cases over graph size and density (uniform or local arcs), weight kind (small and
     2^20..2^31 integers, FP32, FP64), length kind and workgroup size, with the library's own LDS map choice
     (bitmaps, 16-bit bounds, 4-bit codes or LDS distances): distances bit-exact, betweenness within 1e-9."""
@@ -382,8 +384,7 @@ def test_random_configurations(ctx, monkeypatch):
             w = np.exp(rng.normal(0.0, 0.5, m)).astype(np.float32 if kind == "f32" else np.float64)
         c = orc.Csr.from_edges(n, src, dst, w, dtype=w.dtype)
         srcs = np.unique(rng.integers(0, n, 16)).astype(np.uint32)
-        monkeypatch.setenv("NG_BLOCK_SIZE", block)
-        g = ng.Graph(ctx, ng.HostGraph(c.n, c.off, c.tgt, c.w))
+        g = ng.Graph(ctx, ng.HostGraph(c.n, c.off, c.tgt, c.w)).set_options(block=int(block))
         ref = np.stack([orc.sssp(c, int(s), lkind)[0] for s in srcs])
         _, dist = g.sssp(srcs, lkind=lkind, want_dist=True)
         np.testing.assert_array_equal(dist, ref, err_msg=f"case {case}: n={n} m={m} {kind} block {block} lkind {lkind}")
@@ -394,17 +395,16 @@ def test_random_configurations(ctx, monkeypatch):
 
 
 @pytest.mark.parametrize("block", ["64", "256", "1024"])
-def test_many_sources_per_slot(ctx, monkeypatch, block):
+def test_many_sources_per_slot(ctx, block):
     """Few slots, so each runs dozens of sources one after another: every source must find its slot restored
-    (distances, records, bitmaps, lists) by the previous one, whether the previous source's shortest paths
-    formed a tree (continuous weights: the tree layers restore the nodes) or not (small integer weights: ties),
-    or a mix (wide integer weights). Distances bit-exact, betweenness and edge path counts within 1e-9."""
-    monkeypatch.setenv("NG_MAX_SLOTS", "3")
-    monkeypatch.setenv("NG_BLOCK_SIZE", block)
+    (distances, records, second dependencies, bitmaps, lists) by the previous one, whether the previous source's
+    shortest paths formed a tree (continuous weights: the tree layers restore the nodes, parents clear their
+    children's dependencies) or not (small integer weights: ties), or a mix (wide integer weights). Distances
+    bit-exact; betweenness, edge path counts and both adjoint dependencies within 1e-9."""
     for wdtype, seed, wlo, whi in ((np.float64, 51, 1, 9), (np.int32, 52, 1 << 20, 2**31 - 1), (np.int32, 53, 1, 4),
                                    (np.float32, 54, 1, 9)):
         c = rand_graph(1500, 12000, seed, wdtype=wdtype, wlo=wlo, whi=whi)
-        g = dev_graph(ctx, c)
+        g = dev_graph(ctx, c, max_slots=3, block=int(block))
         srcs = np.arange(0, 1500, 7)
         _, dist = g.sssp(srcs, want_dist=True)
         np.testing.assert_array_equal(dist, np.stack([orc.sssp(c, int(s))[0] for s in srcs]))
@@ -412,5 +412,12 @@ def test_many_sources_per_slot(ctx, monkeypatch, block):
         ref = orc.brandes(c, sources=srcs, edge=True)
         np.testing.assert_allclose(g.brandes(srcs), ref["bc"], rtol=1e-9, atol=1e-9)
         np.testing.assert_allclose(g.edge_dependency(srcs), ref["edge"], rtol=1e-9, atol=1e-12)
-        _, dist2 = g.sssp(srcs, want_dist=True)  # the slots are clean after Brandes too
+        # adjoint mode (both dependencies): d(sum d)/dw = edge path counts, -d(sum 1/d)/dw = the 1/d^2-seeded ones
+        g1, g2 = g.adjoint(srcs)
+        assert g.last_stats()["slots"] == 3
+        np.testing.assert_allclose(g1, ref["edge"], rtol=1e-9, atol=1e-12)
+        ref2 = orc.brandes(c, sources=srcs, edge=True, seed=orc.SEED_INV_D2)["edge"]
+        np.testing.assert_allclose(g2, -ref2, rtol=1e-9, atol=1e-12)
+        _, dist2 = g.sssp(srcs, want_dist=True)  # the slots are clean after Brandes and the adjoint too
         np.testing.assert_array_equal(dist2, dist)
+        np.testing.assert_allclose(g.brandes(srcs), ref["bc"], rtol=1e-9, atol=1e-9)
