@@ -88,6 +88,8 @@ typedef struct ng_cohort ng_cohort;
 /* Opens HIP device `device` and creates one stream (default priority). Replaces the per-rank MPI
  * set-up: MPIWrapper::init (external/mpi-wrapper/MPIWrapper.h:88-109). */
 ng_status ng_context_create(int device, ng_context** out);
+/* Releases the caller's handle. Graphs and cohorts created on the context keep it alive: its stream and
+ * workspace are freed when the context and all of them have been destroyed, in any order. */
 void ng_context_destroy(ng_context* ctx);
 
 /* Waits until all work of the context's stream is done (the barrier around timed regions). */

@@ -3,9 +3,14 @@ betweenness centrality. The compute lives in libng_hip.so (HIP kernels behind a 
 include/neurograph_hip.h); this package is its Python binding."""
 
 from ._lib import (  # noqa: F401
+    NG_EDEVICE,
+    NG_EINVAL,
+    NG_ENOMEM,
+    NG_EUNSUPPORTED,
     NG_LEN_ABS,
     NG_LEN_INVERSE,
     NG_LEN_RAW,
+    NG_OK,
     Cohort,
     Context,
     Graph,

@@ -86,7 +86,27 @@ struct ng_context {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int cus = 0;
     Workspace ws;
+    // the creator's handle plus one per live graph and cohort: ng_context_destroy releases the creator's share, and
+    // the context is torn down when the last share goes, so graphs and cohorts may be destroyed after it
+    std::atomic<uint32_t> refs{1};
 };
+
+namespace {
+
+void free_ws(Workspace& w);
+
+void context_release(ng_context* ctx) {
+    if (ctx->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    free_ws(ctx->ws);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+}  // namespace
 
 struct ng_graph {
     ng_context* ctx = nullptr;
@@ -718,13 +738,7 @@ ng_status ng_copy_calibration(ng_context* ctx, uint64_t bytes, double* ms_out) {
 
 void ng_context_destroy(ng_context* ctx) {
     if (!ctx) return;
-    (void)hipSetDevice(ctx->device);
-    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
-    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
-    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    free_ws(ctx->ws);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
+    context_release(ctx);
 }
 
 ng_status ng_graph_create(ng_context* ctx, uint32_t n, uint64_t m, const uint64_t* out_offsets, const uint32_t* out_targets,
@@ -770,17 +784,20 @@ ng_status ng_graph_create(ng_context* ctx, uint32_t n, uint64_t m, const uint64_
         free_view(g->views[0]);
         return st;
     }
+    ctx->refs.fetch_add(1, std::memory_order_relaxed);
     *out = g.release();
     return NG_OK;
 }
 
 void ng_graph_destroy(ng_graph* g) {
     if (!g) return;
-    (void)hipSetDevice(g->ctx->device);
-    (void)hipStreamSynchronize(g->ctx->stream);
+    ng_context* ctx = g->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
     free_view(g->views[0]);
     free_view(g->views[1]);
     delete g;
+    context_release(ctx);
 }
 
 ng_status ng_graph_dist_kind(const ng_graph* g, ng_length_kind lkind, ng_dist_kind* out) {
@@ -1311,16 +1328,19 @@ ng_status ng_cohort_create(ng_context* ctx, ng_graph* const* graphs, uint32_t n_
         c->graphs.push_back(graphs[i]);
         c->delta.push_back(0.0);
     }
+    ctx->refs.fetch_add(1, std::memory_order_relaxed);
     *out = c.release();
     return NG_OK;
 }
 
 void ng_cohort_destroy(ng_cohort* c) {
     if (!c) return;
-    (void)hipSetDevice(c->ctx->device);
-    (void)hipStreamSynchronize(c->ctx->stream);
+    ng_context* ctx = c->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
     cohort_free_device(c);
     delete c;
+    context_release(ctx);
 }
 
 ng_status ng_cohort_brandes(ng_cohort* c, double* bc_out, ng_sssp_summary* summaries_out, double* kernel_ms) {

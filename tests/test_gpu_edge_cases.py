@@ -143,3 +143,37 @@ def test_rejected_inputs(ctx):
         ng.Graph(ctx, ng.HostGraph(2, np.array([0, 1, 1], np.uint64), np.array([2], np.uint32), np.array([1], np.int32)))
     with pytest.raises(ng.NgError):  # offsets that decrease
         ng.Graph(ctx, ng.HostGraph(2, np.array([0, 1, 0], np.uint64), np.zeros(0, np.uint32), np.zeros(0, np.int32)))
+
+
+def test_context_destroyed_before_its_graphs():
+    """ng_context_destroy releases only the caller's handle: graphs and cohorts created on the context keep it
+    alive and keep working until they are destroyed themselves, in any order. This is also the order in which the
+    garbage collector may finalize a reference cycle that holds a context and its graphs."""
+    import gc
+
+    import neurograph_amd as ng
+
+    c = orc.Csr.from_edges(4, [0, 1, 2, 0], [1, 2, 3, 3], [1, 1, 1, 5])
+    own = ng.Context(0)
+    g = dev(own, c)
+    g2 = dev(own, c)
+    co = ng.Cohort(own, [g, g2])
+    own.close()
+    s = g.sssp([0])
+    assert int(s["sum_d_u64"][0]) == 1 + 2 + 3
+    parts, _, _ = co.brandes()
+    np.testing.assert_array_equal(parts[0], [0.0, 2.0, 2.0, 0.0])
+    co.close()
+    g.close()
+    np.testing.assert_array_equal(g2.brandes([0]), [0.0, 2.0, 1.0, 0.0])
+    g2.close()
+
+    class Holder:
+        pass
+
+    h = Holder()
+    h.ctx = ng.Context(0)
+    h.g = dev(h.ctx, c)
+    h.me = h  # a cycle: context and graph are finalized together, in the collector's order
+    del h
+    gc.collect()
