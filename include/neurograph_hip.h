@@ -166,6 +166,22 @@ ng_status ng_distance_histogram(ng_graph* g, const uint32_t* sources, uint32_t n
 ng_status ng_brandes_accumulate(ng_graph* g, const uint32_t* sources, uint32_t n_sources, double delta,
                                 double* bc_inout, ng_sssp_summary* out_opt);
 
+/* ng_brandes_accumulate without any device allocation. No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (tight serving loops, code that must not call hipMalloc). Same arguments, results
+ * and errors as ng_brandes_accumulate; it calls no hipMalloc / hipFree and allocates no host memory. Its working
+ * memory (source slots, per-source summaries, the betweenness reduction) is one fixed device array of 256 MiB per
+ * GPU, allocated when the library's code is loaded on that GPU and held until the process ends; its address is
+ * resolved by ng_context_create. Graphs whose one source slot does not fit it are rejected with NG_EINVAL (a slot
+ * takes about 170 bytes per node plus 16 bytes per arc up to 8 arcs per node: roughly 880 000 nodes at mean
+ * out-degree 20, 2.7 million at mean out-degree 4; the 200 000-node spatial graph gets 4 slots); fewer slots
+ * fit than the GPU could run at once for large graphs, so it is slower there than ng_brandes_accumulate. Every call
+ * re-initialises the array, which keeps the last call's contents afterwards; calls that overlap on one GPU (from
+ * several threads or contexts) race on it and must be serialised by the caller. Sources are run in launches of
+ * at most 16 384. Bucket width and launch shape as in ng_brandes_accumulate, except that 4-bit distance codes are
+ * used only when an earlier call on the graph estimated their scale. */
+ng_status ng_brandes_accumulate_noalloc(ng_graph* g, const uint32_t* sources, uint32_t n_sources, double delta,
+                                        double* bc_inout, ng_sssp_summary* out_opt);
+
 /* Edge-level path counts: edge_inout[m] += sum over sources of sigma(v) * (1 + delta(w)) / sigma(w) for every
  * shortest-path arc (v,w), the per-arc term of BrandesVariant.h:151-154, indexed like out_targets. */
 ng_status ng_edge_dependency(ng_graph* g, const uint32_t* sources, uint32_t n_sources, double delta,

@@ -84,6 +84,7 @@ def lib() -> ctypes.CDLL:
         "ng_graph_get_option": (i32, [vp, i32, P(ctypes.c_int64)]),
         "ng_sssp_summaries": (i32, [vp, vp, u32, i32, dbl, vp, vp]),
         "ng_brandes_accumulate": (i32, [vp, vp, u32, dbl, vp, vp]),
+        "ng_brandes_accumulate_noalloc": (i32, [vp, vp, u32, dbl, vp, vp]),
         "ng_distance_histogram": (i32, [vp, vp, u32, i32, dbl, dbl, u32, vp, vp]),
         "ng_edge_dependency": (i32, [vp, vp, u32, dbl, vp]),
         "ng_apsp_adjoint": (i32, [vp, vp, u32, dbl, vp, vp, vp]),
@@ -304,6 +305,14 @@ class Graph:
         bc = np.zeros(self.n, dtype=np.float64) if bc is None else bc
         out = np.zeros(len(src), dtype=SUMMARY_DTYPE) if summaries else None
         _check(lib().ng_brandes_accumulate(self.h, _ptr(src), len(src), float(delta), _ptr(bc), _ptr(out)))
+        return (bc, out) if summaries else bc
+
+    def brandes_noalloc(self, sources=None, delta: float = 0.0, bc=None, summaries: bool = False):
+        """As brandes(), through the allocation-free entry point (fixed device scratch array per GPU)."""
+        src = self._sources(sources)
+        bc = np.zeros(self.n, dtype=np.float64) if bc is None else bc
+        out = np.zeros(len(src), dtype=SUMMARY_DTYPE) if summaries else None
+        _check(lib().ng_brandes_accumulate_noalloc(self.h, _ptr(src), len(src), float(delta), _ptr(bc), _ptr(out)))
         return (bc, out) if summaries else bc
 
     def edge_dependency(self, sources=None, delta: float = 0.0):

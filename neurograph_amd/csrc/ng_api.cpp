@@ -86,6 +86,7 @@ struct ng_context {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int cus = 0;
     Workspace ws;
+    hipError_t scratch_status = hipSuccess;  // result of resolving the device's scratch array at creation
     // the creator's handle plus one per live graph and cohort: ng_context_destroy releases the creator's share, and
     // the context is torn down when the last share goes, so graphs and cohorts may be destroyed after it
     std::atomic<uint32_t> refs{1};
@@ -697,6 +698,9 @@ ng_status ng_context_create(int device, ng_context** out) {
     NG_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     NG_HIP(hipEventCreate(&ctx->ev0));
     NG_HIP(hipEventCreate(&ctx->ev1));
+    // the allocation-free entry point's scratch array on this device (device current): its address is resolved
+    // here, once per device; a failure is reported by that entry point, the others do not use the array
+    ctx->scratch_status = scratch_setup(device);
     *out = ctx.release();
     return NG_OK;
 }
@@ -951,6 +955,194 @@ ng_status ng_brandes_accumulate(ng_graph* g, const uint32_t* sources, uint32_t n
     NG_HIP(hipMemcpyAsync(tmp.data(), w.bc_out, 8 * static_cast<size_t>(g->n), hipMemcpyDeviceToHost, g->ctx->stream));
     NG_HIP(hipStreamSynchronize(g->ctx->stream));
     for (uint32_t v = 0; v < g->n; ++v) bc_inout[v] += tmp[v];
+    return NG_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+size_t align256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
+
+// Layout of the allocation-free Brandes entry point in the scratch array: the fixed part (work counters, error
+// word, one chunk of sources and summaries, the betweenness output) and `slots` source slots of the batched
+// kernel's Brandes arrays, each array 256-byte aligned. Returns the bytes used; with base != nullptr also points
+// w's arrays into it (second dependencies and gradients stay null: Brandes mode does not use them).
+size_t scratch_layout(char* base, Workspace& w, uint32_t n, uint64_t m, DKind dk, uint32_t slots, uint32_t chunk,
+                      uint32_t pcap, uint32_t logcap) {
+    const size_t ns = static_cast<size_t>(slots) * n;
+    const uint64_t words = (m + 31) / 32;
+    size_t at = 0;
+    auto take = [&](size_t bytes) -> char* {
+        char* p = base ? base + at : nullptr;
+        at += align256(bytes);
+        return p;
+    };
+    w = Workspace{};
+    w.arcs_reached = reinterpret_cast<uint64_t*>(take(8 * 32));
+    w.err = reinterpret_cast<uint32_t*>(take(4));
+    w.sources = reinterpret_cast<uint32_t*>(take(4ull * chunk));
+    w.summaries = take(sizeof(ng_sssp_summary) * chunk);
+    w.bc_out = reinterpret_cast<double*>(take(8ull * n));
+    w.dist = take(dk_size(dk) * ns);
+    w.fstamp = reinterpret_cast<uint32_t*>(take(4 * ns));
+    w.sstamp = reinterpret_cast<uint32_t*>(take(4 * ns));
+    w.lists = reinterpret_cast<uint32_t*>(take(12 * ns));
+    w.pile = take(16ull * 2 * pcap * slots);
+    w.counters = reinterpret_cast<uint32_t*>(take(16ull * slots));
+    w.rec = reinterpret_cast<BcRec*>(take(sizeof(BcRec) * ns));
+    w.order = reinterpret_cast<uint32_t*>(take(4 * ns));
+    w.layers = reinterpret_cast<uint32_t*>(take(4ull * slots * (static_cast<size_t>(n) + 1)));
+    w.bits = reinterpret_cast<uint32_t*>(take(4ull * words * slots));
+    w.log = reinterpret_cast<LogEntry*>(take(sizeof(LogEntry) * static_cast<size_t>(logcap) * slots));
+    w.slots = slots;
+    w.n = n;
+    w.m = m;
+    w.mode_bits = MODE_BC;
+    w.logcap = logcap;
+    w.pcap = pcap;
+    w.words = words;
+    w.summaries_cap = chunk;
+    w.sources_cap = chunk;
+    w.dist_bytes = dk_size(dk);
+    w.dist_kind = dk;
+    w.dist_valid = true;
+    return at;
+}
+
+}  // namespace
+
+extern "C" {
+
+ng_status ng_brandes_accumulate_noalloc(ng_graph* g, const uint32_t* sources, uint32_t n_sources, double delta,
+                                        double* bc_inout, ng_sssp_summary* out_opt) {
+    if (check_graph(g) != NG_OK) return NG_EINVAL;
+    if (n_sources && !sources) return fail(NG_EINVAL, "null sources");
+    if (!bc_inout) return fail(NG_EINVAL, "null betweenness buffer");
+    for (uint32_t i = 0; i < n_sources; ++i) {
+        if (sources[i] >= g->n) {
+            return fail(NG_EINVAL, "source %u is out of range, the graph has %u nodes (Dijkstra.h:66-67)", sources[i], g->n);
+        }
+    }
+    ng_context* ctx = g->ctx;
+    View& v = g->views[0];
+    if (!v.built) return fail(NG_EDEVICE, "the graph's device arrays are missing");
+    if (v.has_zero) return fail(NG_EUNSUPPORTED, "zero-length arcs: Brandes needs non-zero arc weights (BrandesVariant.h:30-31)");
+    if (ctx->scratch_status != hipSuccess) {
+        return fail(NG_EDEVICE, "the scratch array of device %d is unavailable: %s", ctx->device, hipGetErrorString(ctx->scratch_status));
+    }
+    NG_HIP(hipSetDevice(ctx->device));
+    char* const base = static_cast<char*>(scratch_device_ptr(ctx->device));
+    if (base == nullptr) return fail(NG_EDEVICE, "the scratch array of device %d was not resolved", ctx->device);
+    if (n_sources == 0) return NG_OK;
+    const uint32_t n = g->n;
+    const uint64_t m = g->m;
+    const hipStream_t s = ctx->stream;
+    const double d = pick_delta(g, v, 0, 1, delta);
+    if (v.lend_delta != d) {
+        NG_HIP(launch_light_end(v.dk, n, v.off, v.len, d, v.lend, s));
+        v.lend_delta = d;
+    }
+    const int block = pick_block(v, 1, 0, g->opt_block);
+    // 4-bit distance codes only with a scale estimated earlier (estimating it would need workspace memory)
+    const int maps = lds_maps_for(v.dk, MODE_BC, n, block, v.bound_inv_q > 0.0, g->opt_lds_maps) &
+                     (v.bound_inv_q > 0.0 ? ~0 : ~static_cast<int>(MAP_BOUND4));
+    uint64_t slots = static_cast<uint64_t>(batch_blocks_per_cu(v.dk, MODE_BC, n, maps, block)) * static_cast<uint64_t>(ctx->cus);
+    if (g->opt_max_slots > 0) slots = std::min<uint64_t>(slots, g->opt_max_slots);
+    slots = std::min<uint64_t>(slots, n_sources);
+    const uint32_t chunk = std::min(n_sources, kScratchChunk);
+    const uint32_t pcap = wanted_pcap(n, g->opt_pile_capacity);
+    const uint32_t logcap = log_capacity(n, m, g->opt_log_capacity);
+    Workspace w;
+    const size_t fixed = scratch_layout(nullptr, w, n, m, v.dk, 0, chunk, pcap, logcap);
+    const size_t per = scratch_layout(nullptr, w, n, m, v.dk, 1, chunk, pcap, logcap) - fixed;
+    if (fixed + per > kScratchBytes) {
+        return fail(NG_EINVAL,
+                    "graph too large for the allocation-free entry point: %u nodes and %llu arcs need %zu bytes for one "
+                    "source slot, the scratch array holds %zu bytes (use ng_brandes_accumulate)",
+                    n, (unsigned long long)m, fixed + per, static_cast<size_t>(kScratchBytes));
+    }
+    slots = std::min<uint64_t>(slots, (kScratchBytes - fixed) / per);
+    const size_t used = scratch_layout(base, w, n, m, v.dk, static_cast<uint32_t>(slots), chunk, pcap, logcap);
+    if (used > kScratchBytes) return fail(NG_EDEVICE, "scratch layout exceeds the array (%zu bytes)", used);
+    // every call starts from a clean scratch: earlier calls on this GPU (any graph) left their own contents
+    const uint64_t ns = slots * static_cast<uint64_t>(n);
+    NG_HIP(launch_fill_u64(w.arcs_reached, 32, 0, s));
+    NG_HIP(launch_fill_u32(w.err, 1, 0u, s));
+    NG_HIP(launch_fill_dist(v.dk, w.dist, ns, s));
+    NG_HIP(launch_fill_u32(w.fstamp, ns, 0u, s));
+    NG_HIP(launch_fill_u32(w.sstamp, ns, 0u, s));
+    NG_HIP(launch_fill_u32(w.counters, 4 * slots, 1u, s));
+    NG_HIP(launch_rec_init(w.rec, ns, s));
+    NG_HIP(launch_fill_u32(w.bits, w.words * slots, 0u, s));
+    double total_ms = 0.0;
+    uint32_t launches = 0;
+    for (uint32_t b = 0; b < n_sources; b += chunk) {
+        const uint32_t k = std::min(chunk, n_sources - b);
+        NG_HIP(hipMemcpyAsync(w.sources, sources + b, sizeof(uint32_t) * k, hipMemcpyHostToDevice, s));
+        LaunchArgs la{};
+        la.dk = v.dk;
+        la.mode = MODE_BC;
+        la.n = n;
+        la.view = &v;
+        la.sources = w.sources;
+        la.n_sources = k;
+        la.delta = d;
+        la.grid = static_cast<uint32_t>(std::min<uint64_t>(slots, k));
+        la.ws = &w;
+        la.dist_out = nullptr;
+        la.dist_out_kind = dist_kind_of(v.dk);
+        la.maps = maps;
+        la.bd_inv_q = v.bound_inv_q;
+        la.block = block;
+        NG_HIP(hipEventRecord(ctx->ev0, s));
+        const hipError_t le = launch_batch(la, s);
+        if (le != hipSuccess) return fail(NG_EDEVICE, "batched kernel launch failed: %s", hipGetErrorString(le));
+        NG_HIP(hipEventRecord(ctx->ev1, s));
+        const hipError_t se = hipStreamSynchronize(s);
+        if (se != hipSuccess) return fail(NG_EDEVICE, "batched kernel failed: %s", hipGetErrorString(se));
+        float ms = 0.0f;
+        NG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        total_ms += ms;
+        ++launches;
+        uint32_t err = 0;
+        NG_HIP(hipMemcpy(&err, w.err, 4, hipMemcpyDeviceToHost));
+        if (err & ERR_KAHN) return fail(NG_EDEVICE, "tight-arc DAG has a cycle (zero-length arcs?)");
+        if (err & ERR_GUARD) return fail(NG_EDEVICE, "iteration guard hit in the batched kernel");
+        if (err & ERR_OVERFLOW) return fail(NG_EDEVICE, "queue overflow in the batched kernel");
+        if (out_opt) NG_HIP(hipMemcpy(out_opt + b, w.summaries, sizeof(ng_sssp_summary) * k, hipMemcpyDeviceToHost));
+    }
+    NG_HIP(launch_reduce_slots(w.rec, static_cast<uint32_t>(slots), n, n, w.bc_out, s));
+    NG_HIP(hipStreamSynchronize(s));
+    // accumulate into the caller's vector through a fixed host buffer (no host allocation either)
+    double part[4096];
+    for (uint32_t v0 = 0; v0 < n; v0 += 4096) {
+        const uint32_t c = std::min<uint32_t>(4096, n - v0);
+        NG_HIP(hipMemcpy(part, w.bc_out + v0, 8ull * c, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < c; ++i) bc_inout[v0 + i] += part[i];
+    }
+    uint32_t err = 0;
+    unsigned long long ctrs[32] = {};
+    NG_HIP(hipMemcpy(&err, w.err, 4, hipMemcpyDeviceToHost));
+    NG_HIP(hipMemcpy(ctrs, w.arcs_reached, 8 * 32, hipMemcpyDeviceToHost));
+    ng_call_stats& l = g->last;
+    l = ng_call_stats{};
+    l.kernel_ms = total_ms;
+    l.launches = launches;
+    l.slots = static_cast<uint32_t>(slots);
+    l.block = static_cast<uint32_t>(block);
+    l.delta = d;
+    l.arcs_reached = ctrs[0];
+    l.flags = err;
+    l.phases = ctrs[1];
+    l.buckets = ctrs[2];
+    l.light_arcs = ctrs[3];
+    l.settle_arcs = ctrs[4];
+    l.logged_arcs = ctrs[5];
+    l.layers = ctrs[6];
+    l.probes = ctrs[7];
+    l.improvements = ctrs[8];
+    for (int sct = 0; sct < 8; ++sct) l.section_ticks[sct] = ctrs[9 + sct];
     return NG_OK;
 }
 

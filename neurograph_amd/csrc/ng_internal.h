@@ -33,6 +33,13 @@ static_assert(guard_limit(67107302u) == 0xFFFFFFFFu, "saturates where 32-bit ari
 static_assert(guard_limit(1u << 26) == 0xFFFFFFFFu, "2^26 nodes: saturated, not wrapped to 100000");
 static_assert(guard_limit(0xFFFFFFFEu) == 0xFFFFFFFFu, "saturates at 2^32 - 1");
 
+// Scratch of the allocation-free Brandes entry point (ng_brandes_accumulate_noalloc): a file-scope device array
+// of this many bytes, one copy per GPU, held until the process ends; devices 0 .. kScratchMaxDevices - 1.
+constexpr size_t kScratchBytes = 256ull << 20;
+constexpr int kScratchMaxDevices = 64;
+// Sources per launch of the allocation-free entry point (bounds the per-source summaries kept in the scratch).
+constexpr uint32_t kScratchChunk = 16384;
+
 // Device distance element type: lengths and distances are kept in one of these.
 enum class DKind : int { U32 = 0, U64 = 1, F32 = 2, F64 = 3 };
 
@@ -180,6 +187,11 @@ hipError_t launch_distance_histogram(int dist_kind, const void* dist, const uint
 hipError_t launch_fill_u32(uint32_t* p, uint64_t count, uint32_t value, hipStream_t stream);
 hipError_t launch_fill_f64(double* p, uint64_t count, double value, hipStream_t stream);
 hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t stream);  // bytes % 16 == 0
+// Resolves the device address of the scratch array on `device`, which must be current (hipGetSymbolAddress, once per
+// device; later calls return at once). Called by ng_context_create.
+hipError_t scratch_setup(int device);
+// The scratch array's device address on `device`, or nullptr before scratch_setup succeeded there.
+void* scratch_device_ptr(int device);
 // Extra LDS maps beyond the finalized-node bitmap (bit flags of LaunchArgs::maps): bitmaps of each bucket's
 // settled nodes and of the next frontier, 16-bit distance bounds per node, and 4-bit distance codes where the
 // 16-bit bounds do not fit (MAP_BOUND4 needs LaunchArgs::bd_inv_q).

@@ -29,6 +29,7 @@
 
 #include <cfloat>
 #include <cstdint>
+#include <mutex>
 #include <type_traits>
 
 namespace ng {
@@ -1936,6 +1937,40 @@ hipError_t launch_rec_zero_bc(BcRec* rec, uint64_t count, hipStream_t stream) {
     if (count == 0) return hipSuccess;
     hipLaunchKernelGGL(rec_zero_bc_kernel, dim3(static_cast<uint32_t>(grid_for(count))), dim3(256), 0, stream, rec, count);
     return hipGetLastError();
+}
+
+// Scratch of the allocation-free entry point: one file-scope device array, of which every GPU that loads this
+// library holds its own copy (at its own address) until the process ends. In host code the name `scratch` is a
+// host-side stand-in, not a device address: it is never passed to a kernel, a fill or a copy. scratch_setup()
+// resolves the device address once per device (with that device current) and every host-side use goes through
+// scratch_device_ptr().
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kScratchMaxDevices] = {};
+}  // namespace
+
+hipError_t scratch_setup(int device) {
+    if (device < 0 || device >= kScratchMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device] != nullptr) return hipSuccess;
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device) return hipErrorInvalidDevice;  // the caller makes `device` current first
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (ptr == nullptr) return hipErrorInvalidSymbol;
+    g_scratch_ptr[device] = ptr;
+    return hipSuccess;
+}
+
+void* scratch_device_ptr(int device) {
+    if (device < 0 || device >= kScratchMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_ptr[device];
 }
 
 }  // namespace ng
