@@ -251,9 +251,10 @@ def run_adjoint(args):
     d = args.delta  # > 0: a fixed bucket width (and --block) for every pass instead of the tuned shapes
     if args.block:
         g.set_options(block=args.block)
+    shape = {0: (d, args.block), 1: (d, args.block)}  # (bucket width, lanes) per mode: SSSP 0, Brandes/adjoint 1
     if d <= 0.0:
-        g.autotune_shape(sample, mode=1)
-        g.autotune_shape(sample, mode=0)
+        shape[1] = g.autotune_shape(sample, mode=1)
+        shape[0] = g.autotune_shape(sample, mode=0)
     steps = args.steps or 3
     for _ in range(args.warmup):
         g.adjoint(src, delta=d)
@@ -272,6 +273,31 @@ def run_adjoint(args):
         times[name] = (time.perf_counter() - t0) / steps
         kms[name] = k / steps
     copy_calibration(ctx, args.copy_calibration_mib)
+    # one training step of the torch binding's shape (new weights -> forward summaries -> adjoint), with the device
+    # graph rebuilt per step (ng_graph_create) or updated in place (ng_graph_set_weights)
+    rng = np.random.default_rng(2)
+    w_steps = [hg.weights * np.exp(rng.normal(0.0, 1e-3, hg.m)) for _ in range(steps + 1)]
+    def train_rebuild(w):  # the same widths and lanes as the tuned graph
+        gg = ng.Graph(ctx, ng.HostGraph(hg.n, hg.offsets, hg.targets, w))
+        gg.set_options(block=shape[0][1])
+        gg.sssp(src, delta=shape[0][0])
+        gg.set_options(block=shape[1][1])
+        gg.adjoint(src, delta=shape[1][0])
+        gg.close()
+    def train_in_place(w):
+        g.set_weights(w)
+        g.sssp(src, delta=d)
+        g.adjoint(src, delta=d)
+    train = {}
+    for name, fn in (("rebuild", train_rebuild), ("set_weights", train_in_place)):
+        fn(w_steps[-1])
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            fn(w_steps[k])
+        ctx.synchronize()
+        train[name] = (time.perf_counter() - t0) / steps * 1e3
+    g.set_weights(hg.weights)
     value = hg.n / times["adjoint"]
     # adjoint traffic model: B_bc plus the gradient read-modify-write of the two per-arc vectors
     algo = bytes_per_source(hg) * hg.n + 2 * 16.0 * hg.m
@@ -287,6 +313,8 @@ def run_adjoint(args):
         "plain_forward": {"value": round(hg.n / times["forward"], 3), "unit": "sources/s", "kernel_ms": round(kms["forward"], 3)},
         "plain_brandes": {"value": round(hg.n / times["brandes"], 3), "unit": "sources/s", "kernel_ms": round(kms["brandes"], 3)},
         "adjoint_vs_forward": round(times["forward"] / times["adjoint"], 3),
+        "train_step_ms": {"graph_rebuilt_per_step": round(train["rebuild"], 3),
+                          "graph_updated_in_place": round(train["set_weights"], 3)},
         "roofline": roofline(achieved, "adjoint20k"),
     }
     print(json.dumps(res), flush=True)

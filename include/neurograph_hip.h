@@ -113,6 +113,18 @@ void ng_graph_destroy(ng_graph* g);
 /* Distance element type of dist_out for this graph and length kind. */
 ng_status ng_graph_dist_kind(const ng_graph* g, ng_length_kind lkind, ng_dist_kind* out);
 
+/* Replaces every arc weight in place: weights[m] of the graph's weight kind, indexed like out_targets at creation
+ * (the arc structure is unchanged). The device arrays are reused: rows whose (length, target) order the new
+ * weights keep are not re-sorted, and when no row changes order only the lengths are uploaded; integer weights
+ * that change the distance type (32 <-> 64 bits) rebuild the arrays. Tuned bucket widths and launch shapes are
+ * kept (they only affect speed); cohorts holding the graph re-capture on their next replay. A non-finite weight
+ * is NG_EINVAL and leaves the graph unchanged. Lets a training loop (the adjoint's torch binding) reuse one
+ * handle across steps instead of ng_graph_create per step. No reference counterpart. */
+ng_status ng_graph_set_weights(ng_graph* g, const void* weights);
+/* Whether the last ng_graph_create / ng_graph_set_weights changed the row order of the |w| view (1) or only
+ * the lengths (0). */
+ng_status ng_graph_weights_reordered(const ng_graph* g, int* reordered);
+
 /* Launch options of one graph: explicit overrides of the library's automatic launch shape and capacities, for
  * tuning sweeps and for tests that must reach a particular code path. No reference counterpart (the reference
  * has no launch shape). Every option defaults to automatic; results never depend on an option, only speed and

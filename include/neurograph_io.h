@@ -56,6 +56,46 @@ ng_status ng_generate_fixed_degree(uint32_t n, uint32_t k, int32_t wlo, int32_t 
 ng_status ng_generate_spatial(uint32_t n, double box, double lambda, double mean_deg, int wdist, double a, double b,
                               ng_weight_kind wkind, uint64_t seed, ng_host_csr* out);
 
+/* Binary CSR cache: the loaded (and transformed) graph in one file, so that later runs skip the text parse of
+ * ng_load_graph_dir (FileLoader.cpp:179-327 reads every arc line of every rank file). No reference counterpart.
+ * Layout (little-endian): a 64-byte header -- magic "NGCSR\0\0\0", u32 version (1), u32 weight kind, u64 n,
+ * u64 m, u32 flags (bit 0: positions present), u32 zero, u64 key, u64 checksum -- then offsets u64[n+1],
+ * targets u32[m], weights[m] of the weight kind, and positions f64[3n] if present. The checksum is a 64-bit
+ * multiply-xor hash of the header's first 48 bytes and then of each array in turn, in 8-byte words (each array's
+ * tail zero-padded to a word).
+ * `key` is the caller's identifier of the source and load options (the CLI hashes the input path, the prefix and
+ * the transform flags); ng_load_csr refuses a file whose key differs, whose header, size or checksum is wrong,
+ * or whose CSR is malformed (offsets not starting at 0, decreasing, or not ending at m; a target >= n), with
+ * NG_EINVAL and a message naming the reason. */
+ng_status ng_save_csr(const char* path, const ng_host_csr* g, uint64_t key);
+ng_status ng_load_csr(const char* path, uint64_t key, ng_host_csr* out);
+
+/* A source of uniform random numbers in [0, 1) for ng_contract_balls: the reference's RandomNumberGenerator
+ * interface (utility/RandomNumberGenerator.h), so callers (tests) can inject deterministic draws. */
+typedef double (*ng_draw_fn)(void* user);
+
+/* DiameterApproximation's clustering and contraction (metrics/paths/DiameterApproximation.h:71-113, 154-189 mark
+ * centers, 204-284 growing step, 286-295 stabilize, 297-561 contract) for one rank holding the whole graph: in iteration i = 0 .. ceil(log2 n) - 1 every node without a
+ * center draws a number and becomes a center when it is below 2^i / n; balls then grow from their centers over
+ * arcs of weight <= 2r in synchronous rounds (a claim needs d(u) + w <= (i - generation(u) + 1) 2r and must beat
+ * the target's distance; claims of one round apply in order of claiming node, then arc, and a claim at the same
+ * distance as an earlier one of the round loses), until no claim succeeds; nodes with a center become stable; the
+ * loop ends early when every node is stable. The contracted graph has one node per center (in increasing id) and,
+ * for every arc (u, v, w), the arc center(u) -> center(v) of weight w + d(u) + d(v), a later arc between the same
+ * centers replacing an earlier one (in node, then target order). Integer weights only (NG_W_I32); a negative
+ * weight, a node left without a center, or a contracted weight beyond int32 is NG_EINVAL, as the reference
+ * throws. draw == NULL draws from std::mt19937(seed) through std::uniform_real_distribution<double>(0, 1), the
+ * reference's DefaultRandomNumberGenerator seeded with the MPI rank (0 on one rank). centers_out (optional)
+ * receives each node's center. The contracted graph is allocated like every ng_host_csr (ng_host_csr_free). */
+ng_status ng_contract_balls(const ng_host_csr* g, uint64_t r, ng_draw_fn draw, void* user, uint32_t seed, ng_host_csr* out,
+                            uint32_t* centers_out);
+
+/* DiameterApproximation::compute_approximation (DiameterApproximation.h:71-126): ng_contract_balls, then the
+ * diameter of the contracted graph by AllPairsShortestPath::compute_apsp on the device (ng_sssp_summaries over all
+ * its nodes, lengths |w|), plus 2r. The reference's call at DiameterApproximation.h:118 is this inner APSP. */
+ng_status ng_diameter_approximation(ng_context* ctx, const ng_host_csr* g, uint64_t r, ng_draw_fn draw, void* user,
+                                    uint32_t seed, uint64_t* diameter_out, uint32_t* contracted_nodes_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -15,11 +15,14 @@ from ._lib import (  # noqa: F401
     Context,
     Graph,
     HostGraph,
+    contract_balls,
     NgError,
     generate_fixed_degree,
     generate_spatial,
     lib,
+    load_csr,
     load_graph_dir,
+    save_csr,
     version,
     write_graph_dir,
 )

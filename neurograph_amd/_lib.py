@@ -59,6 +59,8 @@ class HostCsrC(ctypes.Structure):
                 ("weights", ctypes.c_void_p), ("positions", ctypes.POINTER(ctypes.c_double))]
 
 
+DRAW_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_void_p)
+
 _lib = None
 
 
@@ -81,6 +83,8 @@ def lib() -> ctypes.CDLL:
         "ng_graph_destroy": (None, [vp]),
         "ng_graph_dist_kind": (i32, [vp, i32, P(i32)]),
         "ng_graph_set_option": (i32, [vp, i32, ctypes.c_int64]),
+        "ng_graph_set_weights": (i32, [vp, vp]),
+        "ng_graph_weights_reordered": (i32, [vp, P(i32)]),
         "ng_graph_get_option": (i32, [vp, i32, P(ctypes.c_int64)]),
         "ng_sssp_summaries": (i32, [vp, vp, u32, i32, dbl, vp, vp]),
         "ng_brandes_accumulate": (i32, [vp, vp, u32, dbl, vp, vp]),
@@ -103,6 +107,10 @@ def lib() -> ctypes.CDLL:
         "ng_write_graph_dir": (i32, [ctypes.c_char_p, P(HostCsrC)]),
         "ng_generate_fixed_degree": (i32, [u32, u32, ctypes.c_int32, ctypes.c_int32, u64, P(HostCsrC)]),
         "ng_generate_spatial": (i32, [u32, dbl, dbl, dbl, i32, dbl, dbl, i32, u64, P(HostCsrC)]),
+        "ng_save_csr": (i32, [ctypes.c_char_p, P(HostCsrC), u64]),
+        "ng_load_csr": (i32, [ctypes.c_char_p, u64, P(HostCsrC)]),
+        "ng_contract_balls": (i32, [P(HostCsrC), u64, DRAW_FN, vp, u32, P(HostCsrC), vp]),
+        "ng_diameter_approximation": (i32, [vp, P(HostCsrC), u64, DRAW_FN, vp, u32, P(u64), P(u32)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -206,6 +214,36 @@ def generate_spatial(n: int, mean_deg: float, box: float = 1000.0, lam: float = 
     return _take_csr(c)
 
 
+def save_csr(path: str, g: HostGraph, key: int = 0) -> None:
+    """Binary CSR cache of a host graph (ng_save_csr); `key` identifies its source and load options."""
+    c = g._as_c()
+    _check(lib().ng_save_csr(path.encode(), ctypes.byref(c), key))
+
+
+def load_csr(path: str, key: int = 0) -> HostGraph:
+    c = HostCsrC()
+    _check(lib().ng_load_csr(path.encode(), key, ctypes.byref(c)))
+    return _take_csr(c)
+
+
+def _draw_fn(draws):
+    """A C draw callback from a Python callable returning floats in [0, 1), or None (the default generator)."""
+    if draws is None:
+        return DRAW_FN(), None
+    return DRAW_FN(lambda _user: float(draws())), None
+
+
+def contract_balls(g: HostGraph, r: int, draws=None, seed: int = 0):
+    """DiameterApproximation's clustering and contraction (ng_contract_balls): (contracted graph, center of each
+    node). draws: a callable returning the next number in [0, 1), or None for std::mt19937(seed)."""
+    fn, _ = _draw_fn(draws)
+    out = HostCsrC()
+    centers = np.zeros(g.n, dtype=np.uint32)
+    c = g._as_c()
+    _check(lib().ng_contract_balls(ctypes.byref(c), int(r), fn, None, int(seed), ctypes.byref(out), _ptr(centers)))
+    return _take_csr(out), centers
+
+
 # ------------------------------------------------------------------------------------------------
 # device handles
 
@@ -223,6 +261,17 @@ class Context:
 
     def synchronize(self):
         _check(lib().ng_context_synchronize(self.h))
+
+    def diameter_approximation(self, g: HostGraph, r: int, draws=None, seed: int = 0):
+        """DiameterApproximation::compute_approximation with the contracted graph's APSP on the device
+        (ng_diameter_approximation): returns (approximate diameter, contracted node count)."""
+        fn, _ = _draw_fn(draws)
+        d = ctypes.c_uint64()
+        k = ctypes.c_uint32()
+        c = g._as_c()
+        _check(lib().ng_diameter_approximation(self.h, ctypes.byref(c), int(r), fn, None, int(seed), ctypes.byref(d),
+                                               ctypes.byref(k)))
+        return d.value, k.value
 
     def copy_calibration(self, nbytes: int) -> float:
         """Runs one plain device copy of nbytes; returns its duration in ms."""
@@ -272,6 +321,18 @@ class Graph:
         for k, v in opts.items():
             _check(lib().ng_graph_set_option(self.h, _OPTIONS[k], int(v)))
         return self
+
+    def set_weights(self, weights) -> bool:
+        """Replaces the arc weights in place (ng_graph_set_weights), same dtype and order as at creation; returns
+        whether the rows had to be re-sorted."""
+        w = np.ascontiguousarray(weights, dtype=self.host.weights.dtype)
+        if w.shape != (self.m,):
+            raise ValueError(f"expected {self.m} weights, got shape {w.shape}")
+        _check(lib().ng_graph_set_weights(self.h, _ptr(w)))
+        self.host = HostGraph(self.n, self.host.offsets, self.host.targets, w, self.host.positions)
+        r = ctypes.c_int()
+        _check(lib().ng_graph_weights_reordered(self.h, ctypes.byref(r)))
+        return bool(r.value)
 
     def get_option(self, name: str) -> int:
         v = ctypes.c_int64()

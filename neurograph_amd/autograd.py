@@ -16,7 +16,7 @@ backward pass recomputes distances and path counts per source batch (no predeces
 
 from __future__ import annotations
 
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 
 import numpy as np
 
@@ -25,12 +25,16 @@ from ._lib import Context, Graph, HostGraph
 
 @dataclass
 class Topology:
-    """The arc structure the weights live on: CSR over out-arcs (offsets[n+1], targets[m])."""
+    """The arc structure the weights live on: CSR over out-arcs (offsets[n+1], targets[m]). It owns one device graph,
+    created on first use and updated in place (ng_graph_set_weights) whenever the weights change, so a training loop
+    does not rebuild the device CSR per step."""
 
     ctx: Context
     n: int
     offsets: np.ndarray
     targets: np.ndarray
+    graph: Graph | None = field(default=None, repr=False)
+    weights: np.ndarray | None = field(default=None, repr=False)  # the weights the device graph holds
 
     @property
     def m(self) -> int:
@@ -43,7 +47,15 @@ class Topology:
 
 
 def _device_graph(topo: Topology, w: np.ndarray) -> Graph:
-    return Graph(topo.ctx, HostGraph(topo.n, topo.offsets, topo.targets, np.ascontiguousarray(w, dtype=np.float64)))
+    """The topology's device graph holding weights w (created once, then updated in place when w differs)."""
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    if topo.graph is None:
+        topo.graph = Graph(topo.ctx, HostGraph(topo.n, topo.offsets, topo.targets, w.copy()))
+        topo.weights = w.copy()
+    elif not np.array_equal(topo.weights, w):
+        topo.graph.set_weights(w)
+        topo.weights = w.copy()
+    return topo.graph
 
 
 def _metrics(summ: np.ndarray, n: int):
@@ -52,7 +64,7 @@ def _metrics(summ: np.ndarray, n: int):
 
 
 def apsp_metrics_and_gradients(topo: Topology, weights: np.ndarray, sources=None):
-    """(CPL, GE, dCPL/dw, dGE/dw) in one adjoint pass, without torch."""
+    """(CPL, GE, dCPL/dw, dGE/dw) in one adjoint pass, without torch (the topology's device graph is reused)."""
     g = _device_graph(topo, weights)
     g1, g2, summ = g.adjoint(sources, summaries=True)
     cpl, ge, reached = _metrics(summ, topo.n)
@@ -67,18 +79,19 @@ def _function():
 
         @staticmethod
         def forward(ctx, weights, topo: Topology, sources):
-            w = weights.detach().to("cpu", torch.float64).numpy()
+            w = weights.detach().to("cpu", torch.float64).numpy().copy()
             g = _device_graph(topo, w)
             summ = g.sssp(sources)
             cpl, ge, reached = _metrics(summ, topo.n)
-            ctx.graph = g
+            ctx.topo = topo
+            ctx.weights = w  # the backward re-runs on these weights (another forward may have moved the graph on)
             ctx.sources = sources
             ctx.reached = reached
             return torch.tensor([cpl, ge], dtype=torch.float64, device=weights.device)
 
         @staticmethod
         def backward(ctx, grad_out):
-            g1, g2 = ctx.graph.adjoint(ctx.sources)
+            g1, g2 = _device_graph(ctx.topo, ctx.weights).adjoint(ctx.sources)
             go = grad_out.detach().to("cpu", torch.float64).numpy()
             grad = (go[0] * g1 + go[1] * g2) / ctx.reached
             return torch.from_numpy(grad).to(grad_out.device), None, None

@@ -28,7 +28,7 @@ COMMON = ["-O3", "-std=c++20", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unus
           f"-I{os.path.join(ROOT, 'include')}"]
 DEVICE = [f"--offload-arch={ARCH}", "-munsafe-fp-atomics", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-SOURCES = ["ng_kernels.hip", "ng_api.cpp", "ng_io.cpp"]
+SOURCES = ["ng_kernels.hip", "ng_api.cpp", "ng_io.cpp", "ng_diameter.cpp"]
 
 
 def _run(cmd):

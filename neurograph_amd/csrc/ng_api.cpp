@@ -126,6 +126,7 @@ struct ng_graph {
     int opt_lds_maps = -1;
     uint32_t opt_pile_capacity = 0;
     uint32_t opt_log_capacity = 0;
+    uint64_t weights_version = 0;  // bumped by ng_graph_set_weights (cohorts re-capture)
 };
 
 // Cohort: all sources of many graphs in one persistent launch, captured once in a hipGraph and replayed.
@@ -153,6 +154,7 @@ struct ng_cohort {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool built = false;
+    std::vector<uint64_t> versions;    // members' weights_version the capture was made against
     ng_call_stats last{};
 };
 
@@ -166,16 +168,12 @@ double weight_at(const ng_graph* g, uint64_t e) {
     }
 }
 
-// Host-side construction of the device CSR for |w| (vi = 0) or 1/|w| (vi = 1).
-// Lengths follow DijkstraCommon::transform_weight (DijkstraCommon.h:36-54).
+// Lengths of the caller's arcs for |w| (inverse = false) or 1/|w| (inverse = true), following
+// DijkstraCommon::transform_weight (DijkstraCommon.h:36-54).
 template <class L>
-ng_status build_view_typed(ng_graph* g, int vi, DKind dk) {
-    View& v = g->views[vi];
-    const uint32_t n = g->n;
-    const uint64_t m = g->m;
-    std::vector<L> len(m);
-    const bool inverse = vi == 1;
-    parallel_for(m, [&](uint64_t b, uint64_t e) {
+void arc_lengths(const ng_graph* g, bool inverse, std::vector<L>& len) {
+    len.resize(g->m);
+    parallel_for(g->m, [&](uint64_t b, uint64_t e) {
         for (uint64_t a = b; a < e; ++a) {
             if (g->wk == NG_W_I32) {
                 const long long aw = std::llabs(static_cast<long long>(reinterpret_cast<const int32_t*>(g->w.data())[a]));
@@ -201,24 +199,71 @@ ng_status build_view_typed(ng_graph* g, int vi, DKind dk) {
             }
         }
     });
-    // sort every row by (length, target): light arcs become a row prefix
-    std::vector<uint32_t> perm(m), stgt(m);
-    std::vector<L> slen(m);
-    parallel_for(n, [&](uint64_t b, uint64_t e) {
-        std::vector<uint32_t> idx;
-        for (uint64_t u = b; u < e; ++u) {
-            const uint64_t r0 = g->off[u], r1 = g->off[u + 1];
-            idx.resize(r1 - r0);
-            std::iota(idx.begin(), idx.end(), static_cast<uint32_t>(r0));
-            std::stable_sort(idx.begin(), idx.end(), [&](uint32_t x, uint32_t y) {
-                if (len[x] != len[y]) return len[x] < len[y];
-                return g->tgt[x] < g->tgt[y];
-            });
-            for (uint64_t k = 0; k < idx.size(); ++k) {
-                perm[r0 + k] = idx[k];
-                stgt[r0 + k] = g->tgt[idx[k]];
-                slen[r0 + k] = len[idx[k]];
+}
+
+// Sorts row u of the view by (length, target), ties in the caller's arc order (a stable sort of the row's arcs).
+template <class L>
+void sort_row(const ng_graph* g, uint32_t u, const std::vector<L>& len, std::vector<uint32_t>& idx, uint32_t* perm) {
+    const uint64_t r0 = g->off[u], r1 = g->off[u + 1];
+    idx.resize(r1 - r0);
+    std::iota(idx.begin(), idx.end(), static_cast<uint32_t>(r0));
+    std::stable_sort(idx.begin(), idx.end(), [&](uint32_t x, uint32_t y) {
+        if (len[x] != len[y]) return len[x] < len[y];
+        return g->tgt[x] < g->tgt[y];
+    });
+    std::copy(idx.begin(), idx.end(), perm + r0);
+}
+
+// Whether row u of perm is still the stable (length, target) order of its arcs under `len`.
+template <class L>
+bool row_in_order(const ng_graph* g, uint32_t u, const std::vector<L>& len, const uint32_t* perm) {
+    for (uint64_t k = g->off[u]; k + 1 < g->off[u + 1]; ++k) {
+        const uint32_t x = perm[k], y = perm[k + 1];
+        if (len[x] < len[y]) continue;
+        if (len[x] > len[y]) return false;
+        if (g->tgt[x] < g->tgt[y]) continue;
+        if (g->tgt[x] > g->tgt[y] || x > y) return false;
+    }
+    return true;
+}
+
+// Host-side construction of the device CSR for |w| (vi = 0) or 1/|w| (vi = 1). Rows are sorted by (length,
+// target) so that the arcs lighter than the bucket width form a prefix. With the view already built in kind dk
+// (a weight update), its device arrays are reused: rows whose order the new lengths keep are not re-sorted, and
+// when no row changes order only the lengths are uploaded.
+template <class L>
+ng_status build_view_typed(ng_graph* g, int vi, DKind dk) {
+    View& v = g->views[vi];
+    const uint32_t n = g->n;
+    const uint64_t m = g->m;
+    const bool update = v.built && v.dk == dk;
+    std::vector<L> len;
+    arc_lengths<L>(g, vi == 1, len);
+    bool reordered = !update;
+    if (!update) {
+        v.perm_host.assign(m, 0u);
+        parallel_for(n, [&](uint64_t b, uint64_t e) {
+            std::vector<uint32_t> idx;
+            for (uint64_t u = b; u < e; ++u) sort_row<L>(g, static_cast<uint32_t>(u), len, idx, v.perm_host.data());
+        });
+    } else {
+        std::atomic<bool> any{false};
+        parallel_for(n, [&](uint64_t b, uint64_t e) {
+            std::vector<uint32_t> idx;
+            for (uint64_t u = b; u < e; ++u) {
+                if (row_in_order<L>(g, static_cast<uint32_t>(u), len, v.perm_host.data())) continue;
+                sort_row<L>(g, static_cast<uint32_t>(u), len, idx, v.perm_host.data());
+                any.store(true, std::memory_order_relaxed);
             }
+        });
+        reordered = any.load();
+    }
+    std::vector<uint32_t> stgt(reordered ? m : 0);
+    std::vector<L> slen(m);
+    parallel_for(m, [&](uint64_t b, uint64_t e) {
+        for (uint64_t k = b; k < e; ++k) {
+            slen[k] = len[v.perm_host[k]];
+            if (reordered) stgt[k] = g->tgt[v.perm_host[k]];
         }
     });
     // statistics over finite lengths
@@ -248,20 +293,26 @@ ng_status build_view_typed(ng_graph* g, int vi, DKind dk) {
             q05 = sample[k];
         }
     }
-    std::vector<uint32_t> off32(static_cast<size_t>(n) + 1);
-    for (uint32_t u = 0; u <= n; ++u) off32[u] = static_cast<uint32_t>(g->off[u]);
-
     NG_HIP(hipSetDevice(g->ctx->device));
-    NG_HIP(hipMalloc(&v.off, sizeof(uint32_t) * (static_cast<size_t>(n) + 1)));
-    NG_HIP(hipMalloc(&v.tgt, sizeof(uint32_t) * std::max<uint64_t>(m, 1)));
-    NG_HIP(hipMalloc(&v.len, sizeof(L) * std::max<uint64_t>(m, 1)));
-    NG_HIP(hipMalloc(&v.perm, sizeof(uint32_t) * std::max<uint64_t>(m, 1)));
-    NG_HIP(hipMalloc(&v.lend, sizeof(uint32_t) * std::max<uint32_t>(n, 1)));
-    NG_HIP(hipMemcpy(v.off, off32.data(), sizeof(uint32_t) * off32.size(), hipMemcpyHostToDevice));
+    if (!update) {
+        std::vector<uint32_t> off32(static_cast<size_t>(n) + 1);
+        for (uint32_t u = 0; u <= n; ++u) off32[u] = static_cast<uint32_t>(g->off[u]);
+        NG_HIP(hipMalloc(&v.off, sizeof(uint32_t) * (static_cast<size_t>(n) + 1)));
+        NG_HIP(hipMalloc(&v.tgt, sizeof(uint32_t) * std::max<uint64_t>(m, 1)));
+        NG_HIP(hipMalloc(&v.len, sizeof(L) * std::max<uint64_t>(m, 1)));
+        NG_HIP(hipMalloc(&v.perm, sizeof(uint32_t) * std::max<uint64_t>(m, 1)));
+        NG_HIP(hipMalloc(&v.lend, sizeof(uint32_t) * std::max<uint32_t>(n, 1)));
+        NG_HIP(hipMemcpy(v.off, off32.data(), sizeof(uint32_t) * off32.size(), hipMemcpyHostToDevice));
+    } else {
+        // earlier calls on the context stream may still read the arrays
+        NG_HIP(hipStreamSynchronize(g->ctx->stream));
+    }
     if (m) {
-        NG_HIP(hipMemcpy(v.tgt, stgt.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice));
+        if (reordered) {
+            NG_HIP(hipMemcpy(v.tgt, stgt.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice));
+            NG_HIP(hipMemcpy(v.perm, v.perm_host.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice));
+        }
         NG_HIP(hipMemcpy(v.len, slen.data(), sizeof(L) * m, hipMemcpyHostToDevice));
-        NG_HIP(hipMemcpy(v.perm, perm.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice));
     }
     v.dk = dk;
     v.min_len = fin ? mn : 0.0;
@@ -269,28 +320,42 @@ ng_status build_view_typed(ng_graph* g, int vi, DKind dk) {
     v.mean_len = fin ? sum / static_cast<double>(fin) : 0.0;
     v.q05_len = q05;
     v.has_zero = zero;
-    v.lend_delta = -1.0;
+    v.lend_delta = -1.0;   // light-prefix ends follow the lengths
+    v.bound_inv_q = 0.0;   // the 4-bit code scale is re-estimated from the new distances
     v.built = true;
+    v.reordered_last = reordered;
     return NG_OK;
 }
 
-ng_status ensure_view(ng_graph* g, int vi) {
-    View& v = g->views[vi];
-    if (v.built) return NG_OK;
-    if (vi == 1) return build_view_typed<double>(g, 1, DKind::F64);
+// Distance kind of view vi: 1/|w| in f64; |w| as u32 for integer weights whose longest simple path ((n-1) max|w|)
+// stays below the sentinel, else u64; f32 and f64 weights keep their type.
+DKind wanted_kind(const ng_graph* g, int vi) {
+    if (vi == 1) return DKind::F64;
     switch (g->wk) {
     case NG_W_I32: {
         uint64_t maxw = 0;
         const int32_t* w = reinterpret_cast<const int32_t*>(g->w.data());
         for (uint64_t a = 0; a < g->m; ++a) maxw = std::max<uint64_t>(maxw, static_cast<uint64_t>(std::llabs(static_cast<long long>(w[a]))));
-        // longest simple path <= (n-1) * max|w|; keep one value below the sentinel
         const long double bound = static_cast<long double>(g->n) * static_cast<long double>(maxw);
-        if (bound < static_cast<long double>(0xFFFFFFF0u)) return build_view_typed<uint32_t>(g, 0, DKind::U32);
-        return build_view_typed<uint64_t>(g, 0, DKind::U64);
+        return bound < static_cast<long double>(0xFFFFFFF0u) ? DKind::U32 : DKind::U64;
     }
-    case NG_W_F32: return build_view_typed<float>(g, 0, DKind::F32);
-    default: return build_view_typed<double>(g, 0, DKind::F64);
+    case NG_W_F32: return DKind::F32;
+    default: return DKind::F64;
     }
+}
+
+ng_status build_view(ng_graph* g, int vi, DKind dk) {
+    switch (dk) {
+    case DKind::U32: return build_view_typed<uint32_t>(g, vi, dk);
+    case DKind::U64: return build_view_typed<uint64_t>(g, vi, dk);
+    case DKind::F32: return build_view_typed<float>(g, vi, dk);
+    default: return build_view_typed<double>(g, vi, dk);
+    }
+}
+
+ng_status ensure_view(ng_graph* g, int vi) {
+    if (g->views[vi].built) return NG_OK;
+    return build_view(g, vi, wanted_kind(g, vi));
 }
 
 void free_view(View& v) {
@@ -300,6 +365,7 @@ void free_view(View& v) {
     v.len = nullptr;
     dfree(v.perm);
     dfree(v.lend);
+    std::vector<uint32_t>().swap(v.perm_host);
     v.built = false;
 }
 
@@ -857,6 +923,49 @@ ng_status ng_graph_get_option(const ng_graph* g, ng_graph_option opt, int64_t* v
     return fail(NG_EINVAL, "unknown graph option %d", static_cast<int>(opt));
 }
 
+ng_status ng_graph_set_weights(ng_graph* g, const void* weights) {
+    if (check_graph(g) != NG_OK) return NG_EINVAL;
+    if (g->m && !weights) return fail(NG_EINVAL, "null weights");
+    const size_t wb = g->wk == NG_W_F64 ? 8 : 4;
+    std::vector<unsigned char> old(g->w);
+    if (g->m) std::memcpy(g->w.data(), weights, wb * g->m);
+    for (uint64_t a = 0; a < g->m; ++a) {
+        if (!std::isfinite(weight_at(g, a))) {
+            g->w.swap(old);
+            return fail(NG_EINVAL, "the weight of arc %llu is not a finite number", (unsigned long long)a);
+        }
+    }
+    g->has_negative = false;
+    for (uint64_t a = 0; a < g->m && !g->has_negative; ++a) g->has_negative = weight_at(g, a) < 0.0;
+    for (int vi = 0; vi < 2; ++vi) {
+        View& v = g->views[vi];
+        if (!v.built) continue;
+        const DKind dk = wanted_kind(g, vi);
+        if (dk != v.dk) {  // integer distances outgrew (or fit again) 32 bits: new arrays
+            NG_HIP(hipSetDevice(g->ctx->device));
+            NG_HIP(hipStreamSynchronize(g->ctx->stream));
+            free_view(v);
+        }
+        const double tuned_d[2] = {v.tuned_delta[0], v.tuned_delta[1]};
+        const int tuned_b[2] = {v.tuned_block[0], v.tuned_block[1]};
+        ng_status st = build_view(g, vi, dk);
+        if (st != NG_OK) return st;
+        for (int k = 0; k < 2; ++k) {  // tuned shapes are speed hints; they stay valid for any lengths
+            v.tuned_delta[k] = tuned_d[k];
+            v.tuned_block[k] = tuned_b[k];
+        }
+    }
+    ++g->weights_version;
+    return NG_OK;
+}
+
+ng_status ng_graph_weights_reordered(const ng_graph* g, int* reordered) {
+    if (check_graph(g) != NG_OK) return NG_EINVAL;
+    if (!reordered) return fail(NG_EINVAL, "null output pointer");
+    *reordered = g->views[0].reordered_last ? 1 : 0;
+    return NG_OK;
+}
+
 ng_status ng_sssp_summaries(ng_graph* g, const uint32_t* sources, uint32_t n_sources, ng_length_kind lkind, double delta,
                             ng_sssp_summary* out, void* dist_out) {
     if (check_graph(g) != NG_OK) return NG_EINVAL;
@@ -1353,6 +1462,8 @@ ng_status cohort_build(ng_cohort* c) {
         gtab[gi] = CohortGraph{v.off, v.tgt, v.len, nullptr, v.perm, g->n, 0u, d};
     }
     c->total = c->prefix[G];
+    c->versions.resize(G);
+    for (uint32_t gi = 0; gi < G; ++gi) c->versions[gi] = c->graphs[gi]->weights_version;
     // per-graph light-prefix ends for the cohort's own widths
     NG_HIP(hipMalloc(&c->d_lend, 4 * std::max<size_t>(c->total, 1)));
     for (uint32_t gi = 0; gi < G; ++gi) {
@@ -1465,6 +1576,8 @@ ng_status cohort_replay(ng_cohort* c) {
         ng_status s0 = ensure_ws(ctx->ws, st, c->n_max, c->m_max, c->slots, MODE_BC, c->dk, c->total, c->slots);
         if (s0 != NG_OK) return s0;
         if (ctx->ws.generation != c->ws_generation) c->built = false;
+        for (size_t gi = 0; gi < c->graphs.size() && c->built; ++gi)
+            if (c->graphs[gi]->weights_version != c->versions[gi]) c->built = false;  // lengths changed: new light prefixes
     }
     if (!c->built) {
         ng_status s0 = cohort_build(c);

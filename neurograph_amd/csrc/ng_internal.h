@@ -62,6 +62,8 @@ struct View {
     double lend_delta = -1.0;
     double bound_inv_q = 0.0;  // scale of the 4-bit distance codes: 15 / (1.1 x largest sampled distance); 0 = not estimated
     uint32_t* lend = nullptr;  // [n] end of the light prefix of each row for lend_delta
+    std::vector<uint32_t> perm_host;  // host copy of perm: weight updates re-sort only the rows whose order changes
+    bool reordered_last = false;      // the last build or weight update changed the row order
 };
 
 // Betweenness state of one node in a slot, co-located so that a random touch of a node in the sigma layers

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -533,6 +534,158 @@ ng_status ng_generate_spatial(uint32_t n, double box, double lambda, double mean
     if (st != NG_OK) return st;
     out->positions = static_cast<double*>(std::malloc(3ull * n * sizeof(double)));
     if (out->positions) std::memcpy(out->positions, pos.data(), 3ull * n * sizeof(double));
+    return NG_OK;
+}
+
+// ---------------------------------------------------------------------------------------------------
+// binary CSR cache
+
+}  // extern "C"
+
+namespace {
+
+constexpr char kCsrMagic[8] = {'N', 'G', 'C', 'S', 'R', 0, 0, 0};
+constexpr uint32_t kCsrVersion = 1;
+
+struct CsrHeader {
+    char magic[8];
+    uint32_t version;
+    uint32_t wkind;
+    uint64_t n;
+    uint64_t m;
+    uint32_t flags;  // bit 0: positions present
+    uint32_t zero;
+    uint64_t key;
+    uint64_t checksum;
+};
+static_assert(sizeof(CsrHeader) == 56, "header fields");
+constexpr size_t kCsrHeaderBytes = 64;  // the header padded with zeros
+
+// 64-bit multiply-xor hash in 8-byte words (a zero-padded tail word), continued across buffers.
+struct Hash64 {
+    uint64_t h = 0x6a09e667f3bcc909ull;
+    void add(const void* p, size_t bytes) {
+        const unsigned char* c = static_cast<const unsigned char*>(p);
+        size_t i = 0;
+        for (; i + 8 <= bytes; i += 8) {
+            uint64_t w;
+            std::memcpy(&w, c + i, 8);
+            mix(w);
+        }
+        if (i < bytes) {
+            uint64_t w = 0;
+            std::memcpy(&w, c + i, bytes - i);
+            mix(w ^ (static_cast<uint64_t>(bytes - i) << 56));
+        }
+    }
+    void mix(uint64_t w) {
+        h ^= w + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
+        h *= 0xff51afd7ed558ccdull;
+        h ^= h >> 32;
+    }
+};
+
+uint64_t csr_checksum(const CsrHeader& hd, const ng_host_csr& g) {
+    Hash64 hs;
+    hs.add(&hd, offsetof(CsrHeader, key) + sizeof(uint64_t));  // the header up to and including the key
+    hs.add(g.offsets, (static_cast<size_t>(g.n) + 1) * sizeof(uint64_t));
+    hs.add(g.targets, g.m * sizeof(uint32_t));
+    hs.add(g.weights, g.m * wsize(g.wkind));
+    if (g.positions) hs.add(g.positions, 3ull * g.n * sizeof(double));
+    return hs.h;
+}
+
+}  // namespace
+
+extern "C" {
+
+ng_status ng_save_csr(const char* path, const ng_host_csr* g, uint64_t key) {
+    if (!path || !g) return io_fail(NG_EINVAL, "null argument");
+    if (!g->offsets || (g->m && (!g->targets || !g->weights))) return io_fail(NG_EINVAL, "incomplete CSR");
+    if (g->wkind != NG_W_I32 && g->wkind != NG_W_F32 && g->wkind != NG_W_F64) return io_fail(NG_EINVAL, "unknown weight kind");
+    CsrHeader hd{};
+    std::memcpy(hd.magic, kCsrMagic, 8);
+    hd.version = kCsrVersion;
+    hd.wkind = static_cast<uint32_t>(g->wkind);
+    hd.n = g->n;
+    hd.m = g->m;
+    hd.flags = g->positions ? 1u : 0u;
+    hd.key = key;
+    hd.checksum = csr_checksum(hd, *g);
+    const std::string tmp = std::string(path) + ".tmp";
+    {
+        std::ofstream f(tmp, std::ios::binary | std::ios::trunc);
+        if (!f) return io_fail(NG_EINVAL, "cannot write %s", tmp.c_str());
+        char head[kCsrHeaderBytes] = {};
+        std::memcpy(head, &hd, sizeof hd);
+        f.write(head, kCsrHeaderBytes);
+        f.write(reinterpret_cast<const char*>(g->offsets), static_cast<std::streamsize>((static_cast<size_t>(g->n) + 1) * sizeof(uint64_t)));
+        if (g->m) {
+            f.write(reinterpret_cast<const char*>(g->targets), static_cast<std::streamsize>(g->m * sizeof(uint32_t)));
+            f.write(static_cast<const char*>(g->weights), static_cast<std::streamsize>(g->m * wsize(g->wkind)));
+        }
+        if (g->positions) f.write(reinterpret_cast<const char*>(g->positions), static_cast<std::streamsize>(3ull * g->n * sizeof(double)));
+        f.flush();
+        if (!f) return io_fail(NG_EINVAL, "writing %s failed", tmp.c_str());
+    }
+    std::error_code ec;
+    fs::rename(tmp, path, ec);
+    if (ec) return io_fail(NG_EINVAL, "cannot rename %s to %s: %s", tmp.c_str(), path, ec.message().c_str());
+    return NG_OK;
+}
+
+ng_status ng_load_csr(const char* path, uint64_t key, ng_host_csr* out) {
+    if (!path || !out) return io_fail(NG_EINVAL, "null argument");
+    std::memset(out, 0, sizeof *out);
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return io_fail(NG_EINVAL, "cannot open %s", path);
+    char head[kCsrHeaderBytes] = {};
+    f.read(head, kCsrHeaderBytes);
+    if (f.gcount() != static_cast<std::streamsize>(kCsrHeaderBytes)) return io_fail(NG_EINVAL, "%s: truncated header", path);
+    CsrHeader hd{};
+    std::memcpy(&hd, head, sizeof hd);
+    if (std::memcmp(hd.magic, kCsrMagic, 8) != 0) return io_fail(NG_EINVAL, "%s: not a CSR cache file", path);
+    if (hd.version != kCsrVersion) return io_fail(NG_EINVAL, "%s: cache version %u, this library reads %u", path, hd.version, kCsrVersion);
+    if (hd.wkind > NG_W_F64 || hd.n >= 0xFFFFFFFFull || hd.m >= 0xFFFFFFFFull || (hd.flags & ~1u) != 0 || hd.zero != 0)
+        return io_fail(NG_EINVAL, "%s: malformed header", path);
+    if (hd.key != key) return io_fail(NG_EINVAL, "%s: written for other input or load options (key %llx, expected %llx)", path,
+                                      (unsigned long long)hd.key, (unsigned long long)key);
+    const ng_weight_kind wk = static_cast<ng_weight_kind>(hd.wkind);
+    const size_t payload = (hd.n + 1) * 8 + hd.m * 4 + hd.m * wsize(wk) + ((hd.flags & 1u) ? 3 * hd.n * 8 : 0);
+    std::error_code ec;
+    const uintmax_t fsize = fs::file_size(path, ec);
+    if (ec || fsize != kCsrHeaderBytes + payload) return io_fail(NG_EINVAL, "%s: size does not match its header", path);
+    out->n = static_cast<uint32_t>(hd.n);
+    out->m = hd.m;
+    out->wkind = wk;
+    out->offsets = static_cast<uint64_t*>(std::malloc((hd.n + 1) * sizeof(uint64_t)));
+    out->targets = static_cast<uint32_t*>(std::malloc(std::max<uint64_t>(hd.m, 1) * sizeof(uint32_t)));
+    out->weights = std::malloc(std::max<uint64_t>(hd.m, 1) * wsize(wk));
+    if ((hd.flags & 1u) != 0) out->positions = static_cast<double*>(std::malloc(std::max<uint64_t>(3 * hd.n, 1) * sizeof(double)));
+    if (!out->offsets || !out->targets || !out->weights || ((hd.flags & 1u) && !out->positions)) {
+        ng_host_csr_free(out);
+        return io_fail(NG_ENOMEM, "host allocation failed");
+    }
+    f.read(reinterpret_cast<char*>(out->offsets), static_cast<std::streamsize>((hd.n + 1) * sizeof(uint64_t)));
+    if (hd.m) {
+        f.read(reinterpret_cast<char*>(out->targets), static_cast<std::streamsize>(hd.m * sizeof(uint32_t)));
+        f.read(static_cast<char*>(out->weights), static_cast<std::streamsize>(hd.m * wsize(wk)));
+    }
+    if (out->positions) f.read(reinterpret_cast<char*>(out->positions), static_cast<std::streamsize>(3 * hd.n * sizeof(double)));
+    const char* why = nullptr;
+    if (!f) why = "truncated payload";
+    else if (csr_checksum(hd, *out) != hd.checksum) why = "checksum mismatch (corrupted file)";
+    else if (out->offsets[0] != 0 || out->offsets[hd.n] != hd.m) why = "offsets do not start at 0 and end at m";
+    if (!why) {
+        for (uint64_t u = 0; u < hd.n && !why; ++u)
+            if (out->offsets[u] > out->offsets[u + 1]) why = "offsets decrease";
+        for (uint64_t a = 0; a < hd.m && !why; ++a)
+            if (out->targets[a] >= hd.n) why = "a target is out of range";
+    }
+    if (why) {
+        ng_host_csr_free(out);
+        return io_fail(NG_EINVAL, "%s: %s", path, why);
+    }
     return NG_OK;
 }
 

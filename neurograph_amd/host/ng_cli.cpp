@@ -2,12 +2,12 @@
 // (source/neurograph.cpp:449-778) and MetricStore-compatible output files
 // (source/utility/MetricStore.cpp:284-373), written to <input>/metrics[/<output-directory>].
 //
-//   ng_cli --input DIR --func avg-apsp,avg-apsp-inv,avg-centrality [--apsp-algorithm dijkstra|delta-stepping]
+//   ng_cli --input DIR --func avg-apsp,avg-apsp-inv,avg-centrality,dia-approx [--apsp-algorithm dijkstra|delta-stepping]
 //          [--apsp-delta D] [--apsp-delta-epoch push-check|...|gpu] [--dijkstra-queue priority-queue|...|gpu]
 //          [--centrality-algorithm brandes|path-enumeration|gpu] [--undirected] [--all-weights-one]
 //          [--remove-autapses] [--prefix P] [--output-directory D] [--threads N]
 //          [--weights int|f32|f64] [--device N] [--node-betweenness] [--distance-matrix]
-//          [--distance-histogram-width W] [--distance-histogram-bins B]
+//          [--distance-histogram-width W] [--distance-histogram-bins B] [--csr-cache FILE]
 //
 // Every selector value runs the GPU variant (they compute identical values); the reference's refusal of
 // the bucket queue with avg-apsp-inv is kept (neurograph.cpp:690-693). --weights f32|f64 reads real-valued
@@ -15,6 +15,10 @@
 // --distance-matrix writes all_pairs_distances.bin (row s = distances from node s, little-endian u64 for integer
 // lengths, f32 for f32 weights, f64 otherwise; unreachable = the type's max) and --distance-histogram-width W
 // writes all_pairs_distance_histogram.txt in the reference's histogram text format (MetricStore.cpp:32-40).
+// dia-approx runs DiameterApproximation::compute_approximation(graph, 1) as neurograph.cpp:386-391 does (radius 1,
+// the default generator seeded with the rank, 0 here), the contracted graph's APSP on the GPU, and writes
+// approximate_diameter.txt (MetricStore.cpp:529-543). --csr-cache FILE reads the graph from a binary CSR cache
+// written for the same input and load options, or parses the text files and writes the cache (extension).
 
 #include <algorithm>
 #include <charconv>
@@ -43,6 +47,16 @@ std::string num(double v) {
     return std::string(buf, r.ptr);
 }
 std::string num(std::uint64_t v) { return std::to_string(v); }
+
+// FNV-1a over the text that identifies an input and its load options: the key of its CSR cache.
+std::uint64_t cache_key(const std::string& text) {
+    std::uint64_t h = 0xcbf29ce484222325ull;
+    for (unsigned char c : text) {
+        h ^= c;
+        h *= 0x100000001b3ull;
+    }
+    return h;
+}
 
 std::vector<std::string> split(const std::string& s) {
     std::vector<std::string> out;
@@ -96,13 +110,15 @@ int main(int argc, char** argv) {
         return EXIT_FAILURE;
     }
     const fs::path input = opt["--input"];
-    bool apsp = false, apsp_inv = false, centrality = false;
+    bool apsp = false, apsp_inv = false, centrality = false, dia_approx = false;
     for (const auto& f : split(get("--func", ""))) {
         if (f == "avg-apsp") apsp = true;
         else if (f == "avg-apsp-inv") apsp_inv = true;
         else if (f == "avg-centrality") centrality = true;
+        else if (f == "dia-approx") dia_approx = true;
         else {
-            std::cerr << "The argument \"" << f << "\" is not a valid algorithm of this implementation (avg-apsp, avg-apsp-inv, avg-centrality)\n";
+            std::cerr << "The argument \"" << f
+                      << "\" is not a valid algorithm of this implementation (avg-apsp, avg-apsp-inv, avg-centrality, dia-approx)\n";
             return EXIT_FAILURE;
         }
     }
@@ -132,15 +148,39 @@ int main(int argc, char** argv) {
     fs::path out_dir = input / "metrics";
     if (opt.count("--output-directory")) out_dir /= opt["--output-directory"];
 
+    if (dia_approx && wkind != NG_W_I32) {
+        std::cerr << "dia-approx takes integer weights (--weights int)\n";
+        return EXIT_FAILURE;
+    }
+
+    ng_host_csr csr{};
     try {
-        ng_host_csr csr{};
-        ng_host::check(ng_load_graph_dir(input.c_str(), get("--prefix", "").c_str(), 0, has("--remove-autapses"), has("--undirected"),
-                                         has("--all-weights-one"), wkind, &csr),
-                       "loading the graph");
+        const std::string prefix = get("--prefix", "");
+        bool from_cache = false;
+        std::uint64_t key = 0;
+        if (opt.count("--csr-cache")) {
+            std::error_code ec;
+            const fs::path canon = fs::weakly_canonical(input, ec);
+            key = cache_key((ec ? input : canon).string() + '\n' + prefix + '\n' + std::to_string(has("--remove-autapses")) +
+                            std::to_string(has("--undirected")) + std::to_string(has("--all-weights-one")) + '\n' + wk);
+            if (fs::exists(opt["--csr-cache"])) {
+                const ng_status st = ng_load_csr(opt["--csr-cache"].c_str(), key, &csr);
+                if (st == NG_OK) from_cache = true;
+                else std::cerr << "ignoring the CSR cache: " << ng_last_error() << "\n";
+            }
+        }
+        if (!from_cache) {
+            ng_host::check(ng_load_graph_dir(input.c_str(), prefix.c_str(), 0, has("--remove-autapses"), has("--undirected"),
+                                             has("--all-weights-one"), wkind, &csr),
+                           "loading the graph");
+            if (opt.count("--csr-cache") && ng_save_csr(opt["--csr-cache"].c_str(), &csr, key) != NG_OK) {
+                std::cerr << "could not write the CSR cache: " << ng_last_error() << "\n";
+            }
+        }
         ng_host::Context ctx(std::stoi(get("--device", "0")));
         ng_host::Graph graph(ctx, csr);
-        ng_host_csr_free(&csr);
-        std::cout << "Loaded " << graph.number_nodes() << " nodes and " << graph.number_arcs() << " arcs\n";
+        std::cout << "Loaded " << graph.number_nodes() << " nodes and " << graph.number_arcs() << " arcs"
+                  << (from_cache ? " from the CSR cache" : "") << "\n";
         fs::create_directories(out_dir);
         if (apsp) {
             if (wkind == NG_W_I32) {
@@ -206,9 +246,15 @@ int main(int argc, char** argv) {
                 for (std::size_t v = 0; v < per_node.size(); ++v) f << (v + 1) << "\t" << num(per_node[v]) << "\n";
             }
         }
+        if (dia_approx) {
+            const auto d = ng_host::DiameterApproximation::compute_approximation(ctx, csr, 1);
+            emit(out_dir, "approximate_diameter.txt", "The approximate diameter is: " + num(d) + "\n");
+        }
     } catch (const std::exception& e) {
+        ng_host_csr_free(&csr);
         std::cerr << e.what() << "\n";
         return EXIT_FAILURE;
     }
+    ng_host_csr_free(&csr);
     return EXIT_SUCCESS;
 }

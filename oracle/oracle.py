@@ -30,7 +30,23 @@ function                    restates                                       pinne
 ``delta_stepping``          DeltaStepping.h:36-82, EpochCommon.h:46-49,     reference per-source tables for every
                             PushCheckVariant.h:44-126                       delta in {1,4,10,10000}; equality with
                                                                             ``sssp``; negative weight -> error
+``contract_balls``,         DiameterApproximation.h:71-126, 154-561         the reference's expectations
+``diameter_approximation``  (one rank, Python loops: small graphs)          (test_diameter_approximation.cpp:57-322);
+                                                                            invariants on random graphs (ball
+                                                                            distances >= shortest distances from the
+                                                                            center, within the grown radius; every
+                                                                            contracted arc comes from an original
+                                                                            arc); the all-centers and one-center
+                                                                            closed forms
+``Mt19937Canonical``        RandomNumberGenerator.h (std::mt19937 +          libstdc++'s own draws, compiled and run
+                            uniform_real_distribution<double>)              beside the test; the standard's 10000th
+                                                                            mt19937 output (4123659995)
 ==========================  =============================================  ==========================================
+
+Ties that the reference leaves to unordered_map iteration order (DistributedGraph.cpp:222-230; two balls claiming
+one node at the same distance in one round, several arcs joining the same two centers) are resolved here in node,
+then target order; outside the reference's own expectations such tie cases are "parity unpinned" against the
+reference itself, and pinned between this restatement and the library's host contraction.
 
 Every function is pinned ("parity pinned"); none is left "parity unpinned".
 """
@@ -38,6 +54,7 @@ Every function is pinned ("parity pinned"); none is left "parity unpinned".
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 from dataclasses import dataclass
 
@@ -216,3 +233,115 @@ def delta_stepping(g: Csr, root: int, delta: int):
 
 def hardware_threads() -> int:
     return int(lib().orc_hardware_threads())
+
+
+# ---------------------------------------------------------------------------------------------------------------
+# DiameterApproximation (one rank), restated in plain Python loops for small graphs
+
+
+class Mt19937Canonical:
+    """Draws of std::mt19937(seed) through std::uniform_real_distribution<double>(0, 1) (the reference's
+    DefaultRandomNumberGenerator, utility/RandomNumberGenerator.h). The C++ standard's generate_canonical for 53
+    bits from a 32-bit engine takes two outputs x1, x2 and returns (x1 + x2 * 2^32) / 2^64, or the largest double
+    below 1 if that rounds to 1 ([rand.util.canonical]); numpy's legacy integer seeding of MT19937 is the
+    standard's mersenne_twister_engine seed sequence."""
+
+    def __init__(self, seed: int):
+        self._bits = np.random.RandomState(seed)._bit_generator
+
+    def __call__(self) -> float:
+        x1 = int(self._bits.random_raw())
+        x2 = int(self._bits.random_raw())
+        v = (x1 + x2 * 4294967296.0) / 18446744073709551616.0
+        return v if v < 1.0 else float(np.nextafter(1.0, 0.0))
+
+
+def contract_balls(g: Csr, r: int, draws=None, seed: int = 0):
+    """DiameterApproximation::compute_approximation up to the contraction (DiameterApproximation.h:71-113):
+    mark_as_centers (:154-179), perform_growing_step (:204-284) in synchronous rounds, stabilize_nodes (:286-295),
+    check_all_stable (:191-202), contract (:297-561). Ties of one round (two claims at the same distance) go to the
+    claim of the lower claiming node / earlier arc; several arcs between the same two centers keep the last one in
+    (node, arc) order. Returns (contracted Csr with int32 weights, center of every node, ball distance of every
+    node). Raises OracleError where the reference throws."""
+    if g.w.dtype != np.int32:
+        raise OracleError("integer weights only")
+    n = g.n
+    draws = Mt19937Canonical(seed) if draws is None else draws
+    off = [int(x) for x in g.off]
+    tgt = [int(x) for x in g.tgt]
+    w = [int(x) for x in g.w]
+    if any(x < 0 for x in w):
+        raise OracleError("negative weight")
+    none = -1
+    center = [none] * n
+    dist = [None] * n  # None = unreached (the reference's max distance)
+    gen = [None] * n
+    stable = [False] * n
+    last_iteration = int(math.ceil(math.log2(n))) if n > 0 else 0
+    for it in range(last_iteration):
+        thr = 2.0 ** it / n
+        any_center = False
+        for v in range(n):
+            if center[v] != none:
+                any_center = True
+                continue
+            if draws() >= thr:
+                continue
+            center[v], dist[v], gen[v], stable[v] = v, 0, it, True
+            any_center = True
+        if not any_center:
+            continue
+        while True:  # growing rounds: questions from the state at the round's start, answers in order
+            questions = []
+            for u in range(n):
+                if center[u] == none:
+                    continue
+                limit = (it - gen[u] + 1) * 2 * r
+                for a in range(off[u], off[u + 1]):
+                    t = tgt[a]
+                    if w[a] == 0 or w[a] > 2 * r or stable[t]:
+                        continue
+                    nd = dist[u] + w[a]
+                    if limit < nd or (dist[t] is not None and nd >= dist[t]):
+                        continue
+                    questions.append((t, center[u], nd, gen[u]))
+            changed = False
+            for t, c, nd, gq in questions:
+                if dist[t] is not None and dist[t] <= nd:
+                    continue
+                center[t], dist[t], gen[t], stable[t] = c, nd, gq, False
+                changed = True
+            if not changed:
+                break
+        for v in range(n):
+            if center[v] != none:
+                stable[v] = True
+        if all(stable):
+            break
+    for v in range(n):
+        if center[v] == none:
+            raise OracleError(f"node {v} was reached by no ball")
+    new_id = {}
+    for v in range(n):
+        if center[v] == v:
+            new_id[v] = len(new_id)
+    arcs = {}
+    for u in range(n):
+        for a in range(off[u], off[u + 1]):
+            if w[a] == 0:
+                continue
+            t = tgt[a]
+            nw = w[a] + dist[u] + dist[t]
+            if nw > 2**31 - 1:
+                raise OracleError("contracted weight beyond int32")
+            arcs[(new_id[center[u]], new_id[center[t]])] = nw
+    keys = sorted(arcs)
+    k = len(new_id)
+    small = Csr.from_edges(k, [s for s, _ in keys], [t for _, t in keys], [arcs[x] for x in keys], dtype=np.int32)
+    return small, np.array(center, dtype=np.uint32), dist
+
+
+def diameter_approximation(g: Csr, r: int, draws=None, seed: int = 0) -> int:
+    """The approximate diameter (DiameterApproximation.h:71-126): the contracted graph's APSP diameter plus 2r."""
+    small, _, _ = contract_balls(g, r, draws, seed)
+    return int(apsp(small)["diameter_u64"]) + 2 * r

@@ -218,6 +218,16 @@ def rows_from(block, reps, unreach=0):
     return [(block["sums"][i], block["maxima"][i], block["eff"][i], unreach) for _ in range(reps) for i in range(10)]
 
 
+DIA = "tests/diameter_approximation/test_diameter_approximation.cpp"
+
+
+def dia(*cases):
+    """Diameter approximation expectations: (draws, r, expected, lines). draws "constant0" always returns 0.0 (every
+    node becomes its own center), "single_center" returns 0.0 once and 1.0 afterwards (one center), "default" is the
+    default generator, for which the reference asserts only that two runs agree (expected None)."""
+    return [{"draws": d, "r": r, "expected": e, "source": f"{DIA}:{lines}"} for d, r, e, lines in cases]
+
+
 def main():
     fixtures = OrderedDict()
 
@@ -230,6 +240,7 @@ def main():
         apsp_inverse=apsp_expect(1.7043209876543211, 0.89050315072273489, 4.5, 0, {10: 10}, False,
                                  "tests/apsp/test_apsp.cpp:274-291"),
         brandes_avg={"value": 13.583333333333332, "source": "tests/centrality/test_brandes.cpp:23-34"},
+        diameter_approximation=dia(("constant0", 2, 11, "60-73"), ("single_center", 2, 4, "150-164")),
         per_source={"rows": [(s, m, e, 0) for s, m, e in zip(
             [25, 31, 24, 22, 20, 45, 32, 31, 23, 26], [5, 6, 4, 4, 4, 7, 6, 5, 4, 5],
             [4.3666666, 3.4833333, 4.0, 4.6666666, 4.9166666, 2.1023809, 3.2833333, 3.3166666, 4.0833333, 3.7833333])],
@@ -238,6 +249,8 @@ def main():
         apsp=apsp_expect(1.6, 0.7222222222222222, 3, 0, {10: 10}, False, "tests/apsp/test_apsp.cpp:110-127"),
         apsp_inverse=apsp_expect(1.6, 0.7222222222222222, 3, 0, {10: 10}, False, "tests/apsp/test_apsp.cpp:293-310"),
         brandes_avg={"value": 5.4, "source": "tests/centrality/test_brandes.cpp:36-47"},
+        diameter_approximation=dia(("constant0", 1, 5, "75-88"), ("single_center", 1, 2, "166-180"),
+                                   ("default", 2, None, "182-195")),
         per_source={"rows": [(s, m, e, 0) for s, m, e in zip(
             [16, 13, 16, 13, 13, 15, 13, 17, 14, 14], [3, 2, 3, 2, 2, 2, 2, 3, 2, 3],
             [6.1666666, 7.0, 5.8333333, 7.0, 7.0, 6.0, 7.0, 5.6666666, 6.5, 6.8333333])],
@@ -246,6 +259,7 @@ def main():
         apsp=apsp_expect(1.0, 1.0, 1, 0, {10: 10}, True, "tests/apsp/test_apsp.cpp:129-146"),
         apsp_inverse=apsp_expect(1.0, 1.0, 1, 0, {10: 10}, True, "tests/apsp/test_apsp.cpp:312-329"),
         brandes_avg={"value": 0.0, "source": "tests/centrality/test_brandes.cpp:49-60"},
+        diameter_approximation=dia(("constant0", 2, 5, "90-103")),
         per_source={"rows": [(9, 1, 9.0, 0)] * 10, "eff_tol": 1e-6, "source": "tests/delta_stepping/test_push_check.cpp:77-95"})
     add("negative", g_negative(),
         per_source={"rows": [(10, 5, 1 / 3 + 1 / 2 + 1 / 5, 0), (3, 2, 1 + 1 / 2, 1), (4, 4, 0.25, 2), (0, 0, 0.0, 3)],
@@ -257,17 +271,21 @@ def main():
         apsp_inverse=apsp_expect(1.7130795130795136, 1.2097533633823983, 4.7727272727272725, 0, {40: 40}, False,
                                  "tests/apsp/test_apsp.cpp:331-348"),
         brandes_avg={"value": 112.12194180319423, "source": "tests/centrality/test_brandes.cpp:62-73"},
+        diameter_approximation=dia(("constant0", 2, 44, "105-118"), ("constant0", 2, 44, "197-212")),
         per_source={"rows": rows_from(LARGE, 4), "eff_tol": 1e-6, "source": "tests/delta_stepping/test_push_check.cpp:97-129"})
     add("four_uu", g_four(uu=True),
         apsp=apsp_expect(2.5025641025641026, 0.48384615384615381, 5, 0, {40: 40}, False, "tests/apsp/test_apsp.cpp:167-184"),
         apsp_inverse=apsp_expect(2.5025641025641026, 0.48384615384615381, 5, 0, {40: 40}, False,
                                  "tests/apsp/test_apsp.cpp:350-367"),
         brandes_avg={"value": 58.6, "source": "tests/centrality/test_brandes.cpp:75-86"},
+        diameter_approximation=dia(("constant0", 2, 9, "120-133"), ("constant0", 2, 9, "214-229"),
+                                   ("default", 2, None, "248-263")),
         per_source={"rows": rows_from(LARGE_UU, 4), "eff_tol": 1e-6, "source": "tests/delta_stepping/test_push_check.cpp:131-175"})
     add("full36", g_full_n(36),
         apsp=apsp_expect(1.0, 1.0, 1, 0, {36: 36}, True, "tests/apsp/test_apsp.cpp:186-203"),
         apsp_inverse=apsp_expect(1.0, 1.0, 1, 0, {36: 36}, True, "tests/apsp/test_apsp.cpp:369-386"),
         brandes_avg={"value": 0.0, "source": "tests/centrality/test_brandes.cpp:88-99"},
+        diameter_approximation=dia(("constant0", 2, 5, "135-148"), ("constant0", 2, 5, "231-246")),
         per_source={"rows": [(35, 1, 35.0, 0)] * 36, "eff_tol": 1e-6, "source": "tests/delta_stepping/test_push_check.cpp:177-195"})
     add("seven", g_seven(),
         apsp=apsp_expect(17.587448559670783, 0.15517143563687033, 40, 2886, {1: 6, 2: 4, 20: 20, 40: 40}, True,
@@ -275,6 +293,7 @@ def main():
         apsp_inverse=apsp_expect(1.7007328947915260, 1.2430679013912713, 4.7727272727272725, 2886,
                                  {1: 6, 2: 4, 20: 20, 40: 40}, False, "tests/apsp/test_apsp.cpp:603-624"),
         brandes_avg={"value": 74.447050803480930, "source": "tests/centrality/test_brandes.cpp:146-158"},
+        diameter_approximation=dia(("constant0", 2, 44, "265-280"), ("constant0", 2, 44, "314-328")),
         per_source={"rows": per_source_seven(), "eff_tol": 1e-6, "source": "tests/delta_stepping/seven_rank_expectations.h:91-119"})
     add("seven_uu", g_seven(uu=True),
         apsp=apsp_expect(2.4061538461538463, 0.50605128205128700, 5, 2880, {1: 3, 2: 4, 3: 3, 20: 20, 40: 40}, False,
@@ -282,12 +301,14 @@ def main():
         apsp_inverse=apsp_expect(2.4061538461538463, 0.50605128205128700, 5.0, 2880, {1: 3, 2: 4, 3: 3, 20: 20, 40: 40},
                                  False, "tests/apsp/test_apsp.cpp:626-647"),
         brandes_avg={"value": 39.171428571428571, "source": "tests/centrality/test_brandes.cpp:160-171"},
+        diameter_approximation=dia(("constant0", 2, 9, "282-296"), ("constant0", 2, 9, "330-344")),
         per_source={"rows": per_source_seven_uu(), "eff_tol": 1e-6,
                     "source": "tests/delta_stepping/seven_rank_expectations.h:126-159"})
     add("full63", g_full_n(63),
         apsp=apsp_expect(1.0, 1.0, 1, 0, {63: 63}, True, "tests/apsp/test_apsp.cpp:510-528"),
         apsp_inverse=apsp_expect(1.0, 1.0, 1.0, 0, {63: 63}, False, "tests/apsp/test_apsp.cpp:649-666"),
         brandes_avg={"value": 0.0, "source": "tests/centrality/test_brandes.cpp:173-185"},
+        diameter_approximation=dia(("constant0", 2, 5, "298-312"), ("constant0", 2, 5, "346-360")),
         per_source={"rows": [(62, 1, 62.0, 0)] * 63, "eff_tol": 1e-6,
                     "source": "tests/delta_stepping/seven_rank_expectations.h:166-175"})
 

@@ -112,3 +112,103 @@ def test_finite_differences_c5(ctx, c5):
     np.testing.assert_allclose(dcpl[arcs], fd_c, rtol=1e-5, atol=2e-9)
     np.testing.assert_allclose(dge[arcs], fd_e, rtol=1e-5, atol=2e-9)
     assert np.count_nonzero(fd_c[:128]) == 128
+
+
+def test_set_weights_matches_fresh_graph(ctx):
+    """ng_graph_set_weights: results equal a graph created with the new weights and the oracle's; scaling every
+    weight by a positive constant keeps every row's order (lengths uploaded only), new random weights re-sort rows;
+    integer weights crossing the 32-bit distance bound switch the distance type and back; non-finite weights are
+    refused and leave the graph unchanged."""
+    import neurograph_amd as ng
+
+    rng = np.random.default_rng(3)
+    n, m = 3000, 30000
+    src, dst = rng.integers(0, n, m), rng.integers(0, n, m)
+    c0 = orc.Csr.from_edges(n, src, dst, np.exp(rng.normal(0, 0.5, m)), dtype=np.float64)
+    g = ng.Graph(ctx, ng.HostGraph(c0.n, c0.off, c0.tgt, c0.w))
+    srcs = np.arange(0, n, 37, dtype=np.uint32)
+    for w, reorder in ((c0.w * 2.5, False), (np.exp(rng.normal(0, 0.5, m)), True), (c0.w.copy(), True)):
+        assert g.set_weights(w) == reorder
+        c = orc.Csr(n, c0.off, c0.tgt, np.ascontiguousarray(w))
+        _, dist = g.sssp(srcs, want_dist=True)
+        np.testing.assert_array_equal(dist, np.stack([orc.sssp(c, int(s))[0] for s in srcs]))
+        fresh = ng.Graph(ctx, ng.HostGraph(n, c0.off, c0.tgt, np.ascontiguousarray(w)))
+        np.testing.assert_array_equal(g.brandes(srcs), fresh.brandes(srcs))  # same lengths, width and slots: bit-identical
+        np.testing.assert_allclose(g.brandes(srcs), orc.brandes(c, sources=srcs)["bc"], rtol=1e-9, atol=1e-9)
+        g1, g2 = g.adjoint(srcs)
+        np.testing.assert_allclose(g1, orc.brandes(c, sources=srcs, edge=True)["edge"], rtol=1e-9, atol=1e-12)
+    bad = c0.w.copy()
+    bad[5] = np.nan
+    with pytest.raises(ng.NgError):
+        g.set_weights(bad)
+    np.testing.assert_array_equal(g.sssp(srcs)["sum_d"], ng.Graph(ctx, ng.HostGraph(n, c0.off, c0.tgt, c0.w)).sssp(srcs)["sum_d"])
+    # integer weights: small (u32 distances) -> up to 2^31 - 1 (u64) -> small again
+    ci = orc.Csr.from_edges(n, src, dst, rng.integers(1, 10, m).astype(np.int32))
+    gi = ng.Graph(ctx, ng.HostGraph(ci.n, ci.off, ci.tgt, ci.w))
+    assert gi.dist_dtype() == np.uint64  # dist_out buffers are u64 for integer lengths either way
+    for w in (rng.integers(1 << 20, 2**31 - 1, m).astype(np.int32), ci.w.copy()):
+        gi.set_weights(w)
+        c = orc.Csr(n, ci.off, ci.tgt, w)
+        _, dist = gi.sssp(srcs, want_dist=True)
+        np.testing.assert_array_equal(dist, np.stack([orc.sssp(c, int(s))[0] for s in srcs]))
+        np.testing.assert_allclose(gi.brandes(srcs), orc.brandes(c, sources=srcs)["bc"], rtol=1e-9, atol=1e-9)
+
+
+def test_set_weights_recaptures_cohort(ctx):
+    import neurograph_amd as ng
+
+    hs = [ng.generate_spatial(n, 10, weights=np.float32, seed=40 + n) for n in (1200, 2500)]
+    gs = [ng.Graph(ctx, h) for h in hs]
+    co = ng.Cohort(ctx, gs)
+    co.brandes()
+    w = (hs[1].weights[::-1]).copy()  # same arcs, weights permuted: new lengths and row orders
+    gs[1].set_weights(w)
+    parts, _, _ = co.brandes()
+    c = orc.Csr(hs[1].n, hs[1].offsets, hs[1].targets, w)
+    np.testing.assert_allclose(parts[1], orc.brandes(c, threads=8)["bc"], rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(parts[1], gs[1].brandes(), rtol=1e-9, atol=1e-9)
+
+
+def test_training_step_reuses_one_graph(ctx, c5):
+    """One gradient step through the torch binding on the C5 graph: the topology keeps one device graph, updated in
+    place; the gradient after the step matches central differences at the new weights and the oracle's edge counts
+    on a 64-source sample."""
+    import torch
+
+    import neurograph_amd as ng
+    from neurograph_amd.autograd import Topology, apsp_metrics
+
+    topo = Topology.of(ctx, c5)
+    rng = np.random.default_rng(17)
+    sources = np.sort(rng.choice(c5.n, size=64, replace=False)).astype(np.uint32)
+    w = torch.tensor(c5.weights, dtype=torch.float64, requires_grad=True)
+    y = apsp_metrics(w, topo, sources)
+    handle = topo.graph.h.value
+    y[0].backward()
+    with torch.no_grad():
+        w -= 0.05 * w.grad / w.grad.abs().max()
+    w.grad = None
+    y2 = apsp_metrics(w, topo, sources)
+    assert topo.graph.h.value == handle
+    (g,) = torch.autograd.grad(y2[0], w)
+    wn = w.detach().numpy().copy()
+    c = orc.Csr(c5.n, c5.offsets, c5.targets, wn)
+    ap = orc.apsp(c, sources=sources, threads=8)
+    reached = len(sources) * (c5.n - 1) - ap["n_disconnected"]  # CPL averages over the sources' reached pairs
+    assert float(y2.detach()[0]) == pytest.approx(ap["sum_d"] / reached, rel=1e-12)
+    np.testing.assert_allclose(g.numpy() * reached, orc.brandes(c, sources=sources, edge=True, threads=8)["edge"],
+                               rtol=1e-9, atol=1e-9)
+    used = np.flatnonzero(g.numpy() != 0.0)
+    arcs = rng.choice(used, size=16, replace=False)
+    h = 1e-6
+    for e in arcs:
+        vals = []
+        for sgn in (1.0, -1.0):
+            ww = wn.copy()
+            ww[e] += sgn * h
+            gg = ng.Graph(ctx, ng.HostGraph(c5.n, c5.offsets, c5.targets, ww))
+            _, d = gg.sssp(sources, want_dist=True)
+            vals.append(host_sums(d)[0])
+            gg.close()
+        fd = (vals[0] - vals[1]) / (2 * h) / reached
+        assert float(g[e]) == pytest.approx(fd, rel=1e-5, abs=2e-9)

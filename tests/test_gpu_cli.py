@@ -115,3 +115,53 @@ def test_cli_distance_matrix_and_histogram(tmp_path):
     offdiag = ref[~np.eye(hg.n, dtype=bool)]
     fin = offdiag[offdiag != np.iinfo(np.uint64).max]
     np.testing.assert_array_equal(counts, np.bincount(np.minimum(fin, 47).astype(np.int64), minlength=48))
+
+
+def test_cli_dia_approx(tmp_path, reference_fixtures):
+    """--func dia-approx: radius 1 and the default generator seeded with rank 0 (neurograph.cpp:386-391), written as
+    approximate_diameter.txt (MetricStore.cpp:529-543); the value equals the oracle's approximation."""
+    import neurograph_amd as ng
+    from tests.test_diameter import ring_graph
+
+    for hg in (ng.HostGraph(*(lambda c: (c.n, c.off, c.tgt, c.w))(csr_from_fixture(reference_fixtures["fixtures"]["standard_uu"]))),
+               ring_graph(3000, 5, 1, 1, 7)):
+        d = str(tmp_path / f"dia{hg.n}")
+        ng.write_graph_dir(d, hg)
+        r = run_cli("--input", d, "--func", "dia-approx")
+        assert r.returncode == 0, r.stderr
+        text = open(os.path.join(d, "metrics", "approximate_diameter.txt")).read()
+        got = int(re.fullmatch(r"The approximate diameter is: (\d+)\n", text).group(1))
+        assert got == orc.diameter_approximation(orc.Csr(hg.n, hg.offsets, hg.targets, hg.weights), 1, None, 0)
+    r = run_cli("--input", d, "--func", "dia-approx", "--weights", "f64")
+    assert r.returncode != 0
+
+
+def test_cli_csr_cache(tmp_path):
+    """--csr-cache: the first run parses the text files and writes the cache, the second reads it and writes the same
+    output; other load options (a different key) make the CLI parse again and replace the cache."""
+    import neurograph_amd as ng
+
+    hg = ng.generate_spatial(4000, 12, weights=np.float64, seed=21)
+    d = str(tmp_path / "cached")
+    ng.write_graph_dir(d, hg)
+    cache = str(tmp_path / "g.ngcsr")
+    outs = []
+    for run in range(2):
+        r = run_cli("--input", d, "--func", "avg-apsp,avg-centrality", "--weights", "f64", "--csr-cache", cache,
+                    "--output-directory", f"run{run}")
+        assert r.returncode == 0, r.stderr
+        assert ("from the CSR cache" in r.stdout) == (run == 1)
+        outs.append([open(os.path.join(d, "metrics", f"run{run}", f)).read()
+                     for f in ("all_pairs_shortest_paths.txt", "average_betweenness_centrality.txt")])
+    assert outs[0] == outs[1]
+    r = run_cli("--input", d, "--func", "avg-apsp", "--weights", "f64", "--undirected", "--csr-cache", cache)
+    assert r.returncode == 0 and "from the CSR cache" not in r.stdout and "other input or load options" in r.stderr
+    # Brandes from a cache-loaded graph against the oracle (the CLI's key depends on the input path, so the cache
+    # is written here through the library with a key of its own)
+    ng.save_csr(str(tmp_path / "k.ngcsr"), hg, key=5)
+    back = ng.load_csr(str(tmp_path / "k.ngcsr"), key=5)
+    ctx = ng.Context(0)
+    g = ng.Graph(ctx, back)
+    src = np.arange(0, hg.n, 13, dtype=np.uint32)
+    c = orc.Csr(hg.n, hg.offsets, hg.targets, hg.weights)
+    np.testing.assert_allclose(g.brandes(src), orc.brandes(c, sources=src, threads=8)["bc"], rtol=1e-9, atol=1e-9)

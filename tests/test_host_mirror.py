@@ -1,6 +1,9 @@
 """The C++20 host mirror of the reference's drivers (neurograph_amd/host/ng_variants.hpp) compiles with the host
 compiler against include/*.h, links libng_hip.so, and reports library errors by throwing ng_host::Exception,
-as the reference throws utility::Exception (Exception.hpp:76). CPU only: no device call succeeds here."""
+as the reference throws utility::Exception (Exception.hpp:76). It also drives the reference's P-rank loop shape on
+the host: Dijkstra APSP over each rank's own sources followed by the reductions (AllPairsShortestPath.h:130-162),
+and delta-stepping APSP over all N global sources on every rank with no reduction (AllPairsShortestPath.h:197-247).
+CPU only: no device call succeeds here."""
 
 import os
 import subprocess
@@ -20,7 +23,7 @@ PROGRAM = textwrap.dedent(r"""
     int main() {
         // selector enums carry the GPU variant next to the reference's values (Types.h:36-60)
         static_assert(static_cast<int>(ng_host::DijkstraQueueType::Gpu) == 3);
-        static_assert(static_cast<int>(ng_host::DeltaSteppingEpochType::GpuMultiSource) == 5);
+        static_assert(static_cast<int>(ng_host::DeltaSteppingEpochType::Gpu) == 5);
         static_assert(static_cast<int>(ng_host::BetweennessCentralityAlgorithm::Gpu) == 2);
         if (std::strncmp(ng_version(), "ng_hip", 6) != 0) return 2;
         // aggregation of per-source summaries, compute_apsp_impl (AllPairsShortestPath.h:130-162)
@@ -32,6 +35,42 @@ PROGRAM = textwrap.dedent(r"""
         if (r.number_disconnected_pairs != 4 || r.diameter != 3) return 3;
         if (r.average_shortest_path_length != 10.0 / 8.0) return 4;
         if (r.cluster_sizes.at(4) != 1 || r.cluster_sizes.at(3) != 1 || r.cluster_sizes.at(1) != 1) return 5;
+        // P-rank shape on the reference's standard graph, with its per-source table (Sigma d, max d, efficiency;
+        // test_push_check.cpp:27-50) standing in for the device results: ranks own {0..2}, {3..6}, {7..9}
+        const std::uint64_t sum_d[10] = {25, 31, 24, 22, 20, 45, 32, 31, 23, 26};
+        const std::uint64_t max_d[10] = {5, 6, 4, 4, 4, 7, 6, 5, 4, 5};
+        auto summary = [&](std::uint32_t s) {
+            return ng_sssp_summary{static_cast<double>(sum_d[s]), sum_d[s], static_cast<double>(max_d[s]), max_d[s], 1.0, 0};
+        };
+        std::vector<ng_host::ApspPartial<std::uint64_t>> parts;
+        std::vector<int> seen(10, 0);
+        for (std::uint32_t rank = 0; rank < 3; ++rank) {
+            const ng_host::RankSlice rs{{0, 3, 7, 10}, rank};
+            ng_host::ApspPartial<std::uint64_t> p;
+            for (std::uint32_t s : rs.local_sources()) {
+                ++seen[s];
+                p.add(summary(s), rs.total());
+            }
+            parts.push_back(p);
+            // delta-stepping: every rank runs all N global sources and its result is already complete
+            const auto all = rs.all_sources();
+            if (all.size() != 10) return 8;
+            ng_host::ApspPartial<std::uint64_t> d;
+            for (std::uint32_t s : all) d.add(summary(s), rs.total());
+            const auto dr = d.finish(rs.total());
+            if (dr.average_shortest_path_length != 3.1 || dr.diameter != 7 || dr.cluster_sizes.at(10) != 10) return 9;
+        }
+        for (int c : seen)
+            if (c != 1) return 10;  // the local sources partition the nodes
+        const auto reduced = ng_host::ApspPartial<std::uint64_t>::reduce(parts, 10);
+        if (reduced.average_shortest_path_length != 3.1 || reduced.diameter != 7 || reduced.cluster_sizes.at(10) != 10) return 11;
+        // reducing the delta-stepping results again would count every source once per rank (3 x 279 / 90)
+        {
+            std::vector<ng_host::ApspPartial<std::uint64_t>> thrice(3);
+            for (auto& t : thrice)
+                for (std::uint32_t s = 0; s < 10; ++s) t.add(summary(s), 10);
+            if (ng_host::ApspPartial<std::uint64_t>::reduce(thrice, 10).average_shortest_path_length != 837.0 / 90.0) return 12;
+        }
         // without a usable device the context constructor throws the library's message
         try {
             ng_host::Context ctx(12345);
