@@ -421,3 +421,41 @@ def test_many_sources_per_slot(ctx, block):
         _, dist2 = g.sssp(srcs, want_dist=True)  # the slots are clean after Brandes and the adjoint too
         np.testing.assert_array_equal(dist2, dist)
         np.testing.assert_allclose(g.brandes(srcs), ref["bc"], rtol=1e-9, atol=1e-9)
+
+
+@pytest.mark.parametrize("block", [256, 1024])
+def test_log_overflow_falls_back_to_row_scan(ctx, block):
+    """A tight-arc log too small for the candidates (NG_OPT_LOG_CAPACITY 64): sources that overflow it rescan every
+    row of their reached nodes instead (flags bit 2). Trees (continuous weights), ties (small integer weights) and
+    mixes: distances bit-exact, betweenness, edge path counts and both adjoint dependencies within 1e-9."""
+    for wdtype, seed, wlo, whi in ((np.float64, 61, 1, 9), (np.int32, 62, 1, 3), (np.int32, 63, 1 << 20, 2**31 - 1)):
+        c = rand_graph(3000, 30000, seed, wdtype=wdtype, wlo=wlo, whi=whi)
+        g = dev_graph(ctx, c, log_capacity=64, block=block)
+        srcs = np.arange(0, 3000, 31)
+        ref = orc.brandes(c, sources=srcs, edge=True, want_dist=True)
+        bc = g.brandes(srcs)
+        assert g.last_stats()["flags"] & 2
+        np.testing.assert_allclose(bc, ref["bc"], rtol=1e-9, atol=1e-9)
+        np.testing.assert_allclose(g.edge_dependency(srcs), ref["edge"], rtol=1e-9, atol=1e-12)
+        g1, g2 = g.adjoint(srcs)
+        np.testing.assert_allclose(g1, ref["edge"], rtol=1e-9, atol=1e-12)
+        ref2 = orc.brandes(c, sources=srcs, edge=True, seed=orc.SEED_INV_D2)["edge"]
+        np.testing.assert_allclose(g2, -ref2, rtol=1e-9, atol=1e-12)
+        _, dist = g.sssp(srcs, want_dist=True)
+        np.testing.assert_array_equal(dist, ref["dist"])
+        # the default capacity does not overflow on these graphs: same values
+        g0 = dev_graph(ctx, c, block=block)
+        np.testing.assert_allclose(g0.brandes(srcs), bc, rtol=1e-12, atol=1e-12)
+        assert not (g0.last_stats()["flags"] & 2)
+
+
+def test_graph_options_are_validated(ctx):
+    import neurograph_amd as ng
+
+    c = rand_graph(500, 3000, 71)
+    g = dev_graph(ctx, c)
+    for name, bad in (("block", 128), ("lds_maps", 32), ("lds_maps", -2), ("max_slots", -1)):
+        with pytest.raises(ng.NgError) as e:
+            g.set_options(**{name: bad})
+        assert e.value.status == ng.NG_EINVAL
+    assert g.get_option("lds_maps") == -1 and g.get_option("block") == 0
