@@ -79,6 +79,8 @@ typedef struct {
                                   advances, summaries, tight-arc verification, sigma layers, dependency
                                   layers, slot reset), summed over workgroups; zero unless the library was
                                   built with -DNG_SECTION_TIMERS */
+    uint64_t tree_sources;   /* betweenness modes: sources whose tight arcs formed a shortest-path tree (every sigma
+                                1: the tree layers ran) */
 } ng_call_stats;
 
 typedef struct ng_context ng_context;

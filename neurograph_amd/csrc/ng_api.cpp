@@ -407,21 +407,21 @@ int lds_maps_for(DKind dk, int mode, uint32_t n, int block, bool allow4, int for
     const int b = allow4 ? (MAP_BOUND | MAP_BOUND4) : static_cast<int>(MAP_BOUND);
     if (forced >= 0) {
         int maps = forced & (allow4 ? 31 : 23);
-        if ((maps & MAP_LDIST) != 0 && (lds_plan_maps(dk, n, maps) & MAP_LDIST) == 0) maps &= ~MAP_LDIST;
+        if ((maps & MAP_LDIST) != 0 && (lds_plan_maps(dk, n, maps, block) & MAP_LDIST) == 0) maps &= ~MAP_LDIST;
         return maps;
     }
     const int base = batch_blocks_per_cu(dk, mode, n, 0, block);
     // graphs small enough: the distances themselves in LDS (256- and 1024-lane workgroups)
     if (block == 256 || block == 1024)
         for (int maps : {MAP_LDIST | MAP_SORTED | MAP_FRONTIER, MAP_LDIST | MAP_FRONTIER, MAP_LDIST | MAP_SORTED, static_cast<int>(MAP_LDIST)})
-            if (lds_plan_maps(dk, n, maps) == maps && batch_blocks_per_cu(dk, mode, n, maps, block) >= base) return maps;
+            if (lds_plan_maps(dk, n, maps, block) == maps && batch_blocks_per_cu(dk, mode, n, maps, block) >= base) return maps;
     // 16-bit bounds where they fit, else 4-bit codes (also where 16-bit bounds would cost workgroups: on a
     // 50k-node graph at 512 lanes, 4-bit codes keep two workgroups per CU, 12 % faster than no bounds)
     const int b4 = allow4 ? static_cast<int>(MAP_BOUND4) : 0;
     for (int maps : {b | MAP_SORTED | MAP_FRONTIER, b | MAP_FRONTIER, b | MAP_SORTED, b, b4 | MAP_SORTED | MAP_FRONTIER,
                      b4 | MAP_FRONTIER, b4 | MAP_SORTED, b4, MAP_SORTED | MAP_FRONTIER, static_cast<int>(MAP_FRONTIER),
                      static_cast<int>(MAP_SORTED)})
-        if (lds_plan_maps(dk, n, maps) == maps && batch_blocks_per_cu(dk, mode, n, maps, block) >= base) return maps;
+        if (lds_plan_maps(dk, n, maps, block) == maps && batch_blocks_per_cu(dk, mode, n, maps, block) >= base) return maps;
     return 0;
 }
 
@@ -596,7 +596,7 @@ ng_status run_batch(ng_graph* g, const RunSpec& rs, uint32_t* slots_used, double
     // slots: resident workgroups of this kernel, bounded by the sources and by device memory
     const int block = pick_block(v, mode_idx, rs.block, g->opt_block);
     const int maps = lds_maps_for(v.dk, rs.mode, g->n, block, !rs.no_bound4, g->opt_lds_maps);
-    if (lds_plan_bound_kind(v.dk, g->n, maps) == 2 && !(v.bound_inv_q > 0.0)) {
+    if (lds_plan_bound_kind(v.dk, g->n, maps, block) == 2 && !(v.bound_inv_q > 0.0)) {
         st = estimate_bound_scale(g, rs.vi, delta);
         if (st != NG_OK) return st;
     }
@@ -667,6 +667,7 @@ ng_status run_batch(ng_graph* g, const RunSpec& rs, uint32_t* slots_used, double
     NG_HIP(hipMemcpy(&err, w.err, 4, hipMemcpyDeviceToHost));
     NG_HIP(hipMemcpy(ctrs, w.arcs_reached, 8 * 32, hipMemcpyDeviceToHost));
     for (int sct = 0; sct < 8; ++sct) g->last.section_ticks[sct] = ctrs[9 + sct];
+    g->last.tree_sources = ctrs[17];
     const unsigned long long arcs = ctrs[0];
     if (rs.dist_out) {
         NG_HIP(hipMemcpy(rs.dist_out, d_dist_out, dob * static_cast<size_t>(rs.n_sources) * g->n, hipMemcpyDeviceToHost));
@@ -1252,6 +1253,7 @@ ng_status ng_brandes_accumulate_noalloc(ng_graph* g, const uint32_t* sources, ui
     l.probes = ctrs[7];
     l.improvements = ctrs[8];
     for (int sct = 0; sct < 8; ++sct) l.section_ticks[sct] = ctrs[9 + sct];
+    l.tree_sources = ctrs[17];
     return NG_OK;
 }
 
@@ -1615,6 +1617,7 @@ ng_status cohort_replay(ng_cohort* c) {
     l.probes = ctrs[7];
     l.improvements = ctrs[8];
     for (int sct = 0; sct < 8; ++sct) l.section_ticks[sct] = ctrs[9 + sct];
+    l.tree_sources = ctrs[17];
     return NG_OK;
 }
 

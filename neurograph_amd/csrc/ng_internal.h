@@ -199,10 +199,10 @@ void* scratch_device_ptr(int device);
 // 16-bit bounds do not fit (MAP_BOUND4 needs LaunchArgs::bd_inv_q).
 // MAP_LDIST keeps the current source's distances themselves in LDS (256-lane workgroups; replaces the bounds).
 enum : int { MAP_SORTED = 1, MAP_FRONTIER = 2, MAP_BOUND = 4, MAP_BOUND4 = 8, MAP_LDIST = 16 };
-// The maps of `maps` that fit next to the kernel's static LDS for n nodes of distance kind dk, and the bound
-// kind they give (0: none, 1: 16-bit, 2: 4-bit).
-int lds_plan_maps(DKind dk, uint32_t n, int maps);
-int lds_plan_bound_kind(DKind dk, uint32_t n, int maps);
+// The maps of `maps` that fit next to the static LDS of the kernel with `block` lanes for n nodes of distance kind
+// dk, and the bound kind they give (0: none, 1: 16-bit, 2: 4-bit).
+int lds_plan_maps(DKind dk, uint32_t n, int maps, int block);
+int lds_plan_bound_kind(DKind dk, uint32_t n, int maps, int block);
 // Resident workgroups per CU of the batched kernel for (dk, mode, extra LDS bitmaps, lanes per workgroup).
 int batch_blocks_per_cu(DKind dk, int mode, uint32_t n, int maps, int block);
 

@@ -27,6 +27,7 @@
 
 #include "ng_internal.h"
 
+#include <algorithm>
 #include <cfloat>
 #include <cstdint>
 #include <mutex>
@@ -223,6 +224,11 @@ struct Lds {
     B runmin;
     unsigned long long red_u[3 * kW];
     double red_d[3 * kW];
+    // wide workgroups: one queue of 64 filter survivors (target, arc, candidate) per wave
+    static constexpr uint32_t kQ = BS >= 512 ? BS : 1;
+    uint32_t q_t[kQ];
+    uint32_t q_e[kQ];
+    B q_n[kQ];
 };
 
 template <int BS>
@@ -402,29 +408,41 @@ __device__ __forceinline__ uint32_t code4(double x) { return x < 15.0 ? static_c
 
 // Arc loop of the relaxation passes (settle passes and frontier phases) in wide workgroups, where few arcs
 // survive the LDS filter (bitmap and bounds). Stage A: UA arcs per lane get their owner, CSR entry and
-// candidate, and are filtered. The survivors of the
-// wave (ranked slot by slot from ballots alone) are moved to consecutive lanes by forward permutes, one full
-// permutation per slot (non-survivors fill the other lanes), so stage B (distance probe, update, apply)
-// runs once for up to 64 survivors instead of once per slot with a few live lanes. With more survivors than
-// active lanes, stage B runs slot by slot as in expand_relax. All active lanes call apply (it uses ballots).
-template <class B>
-__device__ __forceinline__ B permute_lanes(int addr, B x) {
-    if constexpr (sizeof(B) == 8) {
-        const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_ds_permute(addr, static_cast<int>(static_cast<uint32_t>(x))));
-        const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_ds_permute(addr, static_cast<int>(static_cast<uint32_t>(x >> 32))));
-        return (static_cast<B>(hi) << 32) | lo;
-    } else {
-        return static_cast<B>(__builtin_amdgcn_ds_permute(addr, static_cast<int>(x)));
-    }
-}
-
+// candidate, and are filtered. The survivors are appended to the wave's LDS queue (ranked slot by slot from
+// ballots alone), and stage B (distance probe, update, apply) runs only when the next survivors would not fit
+// and once at the end of the tile, one queued survivor per lane: the settle passes keep a few survivors per
+// iteration, so their probes and updates wait for memory once every few iterations instead of every time. More
+// survivors than a queue holds run stage B slot by slot as in expand_relax. The loop runs wave-uniform
+// iterations, so every lane takes part in each stage B (apply uses ballots).
 template <int UA, int BS, class B, class L, class NodeFn, class AddFn, class SkipFn, class ProbeFn, class LowerFn, class ApplyFn>
 __device__ __forceinline__ void expand_relax_compact(const uint32_t* list, uint32_t count, Lds<B, BS>& S, gptr<uint32_t> tgt,
                                                      gptr<L> len, NodeFn node_fn, AddFn add, SkipFn skip, ProbeFn probe,
                                                      LowerFn lower, ApplyFn apply) {
+    static_assert(BS >= 512, "the survivor queues are laid out for wide workgroups");
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = lane_id();
     const unsigned long long below = (1ull << lane) - 1ull;
+    uint32_t* const qt = S.q_t + (tid & ~63u);
+    uint32_t* const qe = S.q_e + (tid & ~63u);
+    B* const qn = S.q_n + (tid & ~63u);
+    uint32_t qc = 0;  // queued survivors of this wave (wave-uniform)
+    // stage B for the queued survivors, one per lane
+    auto drain = [&]() {
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");  // the wave's queue stores before its loads
+        const bool live = lane < qc;
+        uint32_t ct = 0, ce = 0;
+        B cnd = 0;
+        if (live) {
+            ct = qt[lane];
+            ce = qe[lane];
+            cnd = qn[lane];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");  // every lane has read before the queue is refilled
+        const B cur = live ? probe(ct) : B(0);
+        const B old = lower(live, ct, cnd, cur);
+        apply(live, ce, ct, cnd, cur, old);
+        qc = 0;
+    };
     for (uint32_t base = 0; base < count; base += BS) {
         const uint32_t i = base + tid;
         uint32_t v = 0, beg = 0, cnt = 0;
@@ -449,7 +467,8 @@ __device__ __forceinline__ void expand_relax_compact(const uint32_t* list, uint3
         }
         if (tid == 0) S.t_own[nq] = static_cast<uint16_t>(BS - 1);
         __syncthreads();
-        for (uint32_t k0 = tid; k0 < total; k0 += UA * BS) {
+        for (uint32_t kb = tid & ~63u; kb < total; kb += UA * BS) {  // wave-uniform trip count
+            const uint32_t k0 = kb + lane;
             uint32_t st[UA], se[UA], so[UA];
             B snd[UA];
             L sl[UA];
@@ -498,31 +517,19 @@ __device__ __forceinline__ void expand_relax_compact(const uint32_t* list, uint3
             }
             const uint32_t nsv = o[UA];
             if (nsv == 0) continue;
-            // compaction needs a receiving lane per survivor: at most 64, all of them active
-            const unsigned long long need = nsv >= 64 ? ~0ull : ((1ull << nsv) - 1ull);
-            if (nsv <= 64 && (__activemask() & need) == need) {
-                // survivors to lanes [0, nsv): slot j's to [o[j], o[j+1]), the rest of the lanes to the others
-                uint32_t ct = 0, ce = 0;
-                B cnd = 0;
+            if (nsv <= 64) {
+                if (qc + nsv > 64) drain();
+                // slot j's survivors to queue entries [qc + o[j], qc + o[j+1])
 #pragma unroll
                 for (int j = 0; j < UA; ++j) {
-                    const uint32_t c = o[j + 1] - o[j];
-                    const uint32_t dst = sv[j] ? o[j] + static_cast<uint32_t>(__popcll(m[j] & below))
-                                               : (o[j] + c + static_cast<uint32_t>(__popcll(~m[j] & below))) & 63u;
-                    const int addr = static_cast<int>(dst << 2);
-                    const uint32_t rt = static_cast<uint32_t>(__builtin_amdgcn_ds_permute(addr, static_cast<int>(st[j])));
-                    const uint32_t re = static_cast<uint32_t>(__builtin_amdgcn_ds_permute(addr, static_cast<int>(se[j])));
-                    const B rn = permute_lanes<B>(addr, snd[j]);
-                    if (lane >= o[j] && lane < o[j + 1]) {
-                        ct = rt;
-                        ce = re;
-                        cnd = rn;
+                    if (sv[j]) {
+                        const uint32_t q = qc + o[j] + static_cast<uint32_t>(__popcll(m[j] & below));
+                        qt[q] = st[j];
+                        qe[q] = se[j];
+                        qn[q] = snd[j];
                     }
                 }
-                const bool live = lane < nsv;
-                const B cur = live ? probe(ct) : B(0);
-                const B old = lower(live, ct, cnd, cur);
-                apply(live, ce, ct, cnd, cur, old);
+                qc += nsv;
             } else {
                 B cur[UA], old[UA];
 #pragma unroll
@@ -533,6 +540,7 @@ __device__ __forceinline__ void expand_relax_compact(const uint32_t* list, uint3
                 for (int j = 0; j < UA; ++j) apply(sv[j], se[j], st[j], snd[j], cur[j], old[j]);
             }
         }
+        if (qc != 0u) drain();
         __syncthreads();
     }
 }
@@ -652,6 +660,17 @@ __device__ __forceinline__ void for_row_words(const uint32_t* bits, uint32_t b, 
 #ifndef NG_WAVES_PER_EU
 #define NG_WAVES_PER_EU 5
 #endif
+// Arcs per lane in the filter stage of the wide workgroups' arc loop: frontier phases, and settle passes with 4-
+// and 8-byte distances.
+#ifndef NG_PHASE_ARCS
+#define NG_PHASE_ARCS 3
+#endif
+#ifndef NG_SETTLE_ARCS_4B
+#define NG_SETTLE_ARCS_4B 6
+#endif
+#ifndef NG_SETTLE_ARCS_8B
+#define NG_SETTLE_ARCS_8B 3
+#endif
 // LD: the current source's distances live in LDS (graphs small enough; 256-lane workgroups), so probes and
 // updates are LDS operations; otherwise they live in the slot's global array.
 template <class D, int MODE, int BS, bool LD>
@@ -753,6 +772,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
     L delta = a.delta;
     // work counters (uniform per workgroup, except the arc counts which lane 0 accumulates from the tiles)
     unsigned long long c_phases = 0, c_buckets = 0, c_logged = 0, c_layers = 0;
+    unsigned long long c_trees = 0;  // sources whose tight arcs formed a shortest-path tree
     // per-lane counts of one launch (32 bits: a lane handles far fewer than 2^32 arcs per launch)
     uint32_t c_light = 0, c_settle = 0, c_probe = 0, c_improve = 0;
 
@@ -957,7 +977,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                     c_light += cnt;
                 };
                 if constexpr (BS >= 512) {
-                    expand_relax_compact<3, BS>(
+                    expand_relax_compact<NG_PHASE_ARCS, BS>(
                         Fc, nF, S, tgt, len, light_row, [](B dv, L l) -> B { return T::add(dv, l); }, skip_arc, probe_t,
                         lower_nd,
                         [&](bool live, uint32_t e, uint32_t t, B nd, B cur, B old) { apply_core(live, e, t, nd, cur, old, nph); });
@@ -1037,7 +1057,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                     // wide workgroups: compacted survivors (the 256-lane kernel's register budget is too small)
                     // arcs per lane in the filter stage: fewer for 8-byte distances (registers; measured 3 vs 4:
                     // +1 % on spatial50k), more for 4-byte ones (6 vs 4: +0.6 % on spatial200k)
-                    constexpr int kFilterArcs = sizeof(B) == 8 ? 3 : 6;
+                    constexpr int kFilterArcs = sizeof(B) == 8 ? NG_SETTLE_ARCS_8B : NG_SETTLE_ARCS_4B;
                     expand_relax_compact<kFilterArcs, BS>(
                         Sl + settle_from, nS - settle_from, S, tgt, len, settle_row,
                         [](B dv, L l) -> B { return T::add(dv, l); }, skip_arc, probe_t, lower_nd,
@@ -1259,6 +1279,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
             // in-degrees or sums, and the dependency term is 1 + delta(t) (the general formula with sigma = 1,
             // bit for bit). Otherwise a second pass counts the tight in-degrees for the general layers.
             const bool tree = S.cnt[6] + 1u == nS;
+            c_trees += tree ? 1u : 0u;
             // Brandes on a tree pushes dependencies to parents: every delta is a subtree size, an integer, so the
             // atomic sums are exact in any order. The sigma layers record each node's parent beside its order
             // position (in the second frontier list, idle after the distances are final) and clear the bitmap.
@@ -1499,6 +1520,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
         atomicAdd(&a.arcs_reached[2], c_buckets);
         atomicAdd(&a.arcs_reached[5], c_logged);
         atomicAdd(&a.arcs_reached[6], c_layers);
+        atomicAdd(&a.arcs_reached[17], c_trees);
     }
     // per-lane arc counts (each lane counted the rows it loaded)
     auto add64 = [](unsigned long long x, unsigned long long y) { return x + y; };
@@ -1565,9 +1587,20 @@ hipError_t fill(T* p, uint64_t count, T value, hipStream_t stream) {
 // MAP_FRONTIER).
 uint32_t lds_maps(int maps) { return 1u + (maps & MAP_SORTED ? 1u : 0u) + (maps & MAP_FRONTIER ? 1u : 0u); }
 
+// Static LDS of the batched kernel: the expansion tile, reductions and, in wide workgroups, the survivor queues.
+size_t lds_static_bytes(uint32_t dsize, int block) {
+    const bool w = dsize == 8;
+    switch (block) {
+    case 64: return w ? sizeof(Lds<unsigned long long, 64>) : sizeof(Lds<uint32_t, 64>);
+    case 512: return w ? sizeof(Lds<unsigned long long, 512>) : sizeof(Lds<uint32_t, 512>);
+    case 1024: return w ? sizeof(Lds<unsigned long long, 1024>) : sizeof(Lds<uint32_t, 1024>);
+    default: return w ? sizeof(Lds<unsigned long long, 256>) : sizeof(Lds<uint32_t, 256>);
+    }
+}
+
 // Dynamic LDS of the batched kernel for n nodes: the bitmaps (none when they exceed 96 KB), then with
 // MAP_BOUND the 16-bit distance bounds, or with MAP_BOUND4 the 4-bit codes where those do not fit, as long
-// as everything stays within 136 KB (the static LDS takes at most ~21 KB of the 160 KB of a CU). The plan's
+// as everything stays within 136 KB and, with the kernel's static LDS, within the 160 KB of a CU. The plan's
 // maps are the requested ones that fit.
 struct LdsPlan {
     uint32_t bm_words = 0;
@@ -1579,7 +1612,7 @@ struct LdsPlan {
 };
 // dsize: bytes per distance. With MAP_LDIST, the distances (n x dsize, 8-byte aligned) follow the bitmaps
 // instead of the bounds.
-LdsPlan lds_plan(uint32_t n, int maps, uint32_t dsize) {
+LdsPlan lds_plan(uint32_t n, int maps, uint32_t dsize, int block) {
     LdsPlan p;
     const uint32_t w = (n + 31) / 32;
     const size_t bm_bytes = static_cast<size_t>(w) * 4 * lds_maps(maps);
@@ -1587,7 +1620,7 @@ LdsPlan lds_plan(uint32_t n, int maps, uint32_t dsize) {
     p.bm_words = w;
     p.maps = maps & (MAP_SORTED | MAP_FRONTIER);
     p.bytes = bm_bytes;
-    constexpr size_t kBudget = 136 * 1024;
+    const size_t kBudget = std::min<size_t>(136 * 1024, 160 * 1024 - lds_static_bytes(dsize, block) - 256);
     if ((maps & MAP_LDIST) != 0) {
         const uint32_t off = (w * lds_maps(maps) + 1u) & ~1u;
         const size_t bytes = static_cast<size_t>(off) * 4 + static_cast<size_t>(n) * dsize;
@@ -1665,7 +1698,7 @@ hipError_t launch_typed(const LaunchArgs& la, hipStream_t stream) {
     k.err = ws.err;
     k.dist_out = la.dist_out;
     k.dist_out_kind = la.dist_out_kind;
-    const LdsPlan lp = lds_plan(la.n, LD ? la.maps : (la.maps & ~MAP_LDIST), static_cast<uint32_t>(sizeof(B)));
+    const LdsPlan lp = lds_plan(la.n, LD ? la.maps : (la.maps & ~MAP_LDIST), static_cast<uint32_t>(sizeof(B)), BS);
     if (LD && (lp.maps & MAP_LDIST) == 0) return hipErrorInvalidValue;
     k.bm_words = lp.bm_words;
     k.bd_words = lp.bd_words;
@@ -1721,7 +1754,7 @@ template <class D>
 int blocks_per_cu_mode(int mode, uint32_t n, int maps, int block) {
     using B = typename DT<D>::B;
     const bool ld = (block == 256 || block == 1024) && (maps & MAP_LDIST) != 0;
-    const size_t dyn = lds_plan(n, ld ? maps : (maps & ~MAP_LDIST), static_cast<uint32_t>(sizeof(B))).bytes;
+    const size_t dyn = lds_plan(n, ld ? maps : (maps & ~MAP_LDIST), static_cast<uint32_t>(sizeof(B)), block).bytes;
     switch (block) {
     case 64: return blocks_per_cu_bs<D, 64, false>(mode, dyn);
     case 512: return blocks_per_cu_bs<D, 512, false>(mode, dyn);
@@ -1742,11 +1775,11 @@ hipError_t launch_batch(const LaunchArgs& a, hipStream_t stream) {
     return hipErrorInvalidValue;
 }
 
-int lds_plan_maps(DKind dk, uint32_t n, int maps) {
-    return lds_plan(n, maps, (dk == DKind::U32 || dk == DKind::F32) ? 4u : 8u).maps;
+int lds_plan_maps(DKind dk, uint32_t n, int maps, int block) {
+    return lds_plan(n, maps, (dk == DKind::U32 || dk == DKind::F32) ? 4u : 8u, block).maps;
 }
-int lds_plan_bound_kind(DKind dk, uint32_t n, int maps) {
-    return static_cast<int>(lds_plan(n, maps, (dk == DKind::U32 || dk == DKind::F32) ? 4u : 8u).bd_kind);
+int lds_plan_bound_kind(DKind dk, uint32_t n, int maps, int block) {
+    return static_cast<int>(lds_plan(n, maps, (dk == DKind::U32 || dk == DKind::F32) ? 4u : 8u, block).bd_kind);
 }
 
 int batch_blocks_per_cu(DKind dk, int mode, uint32_t n, int maps, int block) {
