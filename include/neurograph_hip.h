@@ -73,7 +73,7 @@ typedef struct {
     uint64_t settle_arcs;    /* arcs scanned in settle passes */
     uint64_t logged_arcs;    /* candidate tight arcs logged (betweenness modes) */
     uint64_t layers;         /* Kahn layers of the tight DAG (betweenness modes) */
-    uint64_t probes;         /* arc-target distance probes issued (arcs minus skipped ones) */
+    uint64_t probes;         /* arc-target distance updates issued (arcs past the LDS filter) */
     uint64_t improvements;   /* successful distance improvements */
     uint64_t section_ticks[8]; /* wall-clock ticks per kernel section (frontier phases, settle passes, bucket
                                   advances, summaries, tight-arc verification, sigma layers, dependency

@@ -283,18 +283,18 @@ __device__ __forceinline__ void expand(const uint32_t* list, uint32_t count, Lds
 }
 
 // Arc loop of the relaxation passes with U arcs in flight per lane: owners are located for all U arcs,
-// then all U CSR loads are issued, then all U distance probes, then all U distance updates (returning
-// atomics), and only then are the results applied, so each lane keeps U independent accesses outstanding
-// at every step instead of one. The owner of arc k lies between the
+// then all U CSR loads are issued, then all U arcs are filtered and all U distance updates (returning
+// atomics) issued, and only then are the results applied, so each lane keeps U independent accesses
+// outstanding at every step instead of one. The owner of arc k lies between the
 // owners of the first arcs of its chunk and of the next chunk (a per-tile table of at most 2 BS chunks), so
 // the search is short instead of log2(BS) dependent LDS reads. apply() runs on every active lane (it may
 // append to queues with wave ballots), with ok = false for padding slots.
 #ifndef NG_UNROLL
 #define NG_UNROLL 2
 #endif
-template <int U, int BS, class B, class L, class NodeFn, class ProbeFn, class LowerFn, class ApplyFn>
+template <int U, int BS, class B, class L, class NodeFn, class FilterFn, class UpdateFn, class ApplyFn>
 __device__ __forceinline__ void expand_relax(const uint32_t* list, uint32_t count, Lds<B, BS>& S, gptr<uint32_t> tgt,
-                                             gptr<L> len, NodeFn node_fn, ProbeFn probe, LowerFn lower, ApplyFn apply) {
+                                             gptr<L> len, NodeFn node_fn, FilterFn filter, UpdateFn update, ApplyFn apply) {
     const uint32_t tid = threadIdx.x;
     for (uint32_t base = 0; base < count; base += BS) {
         const uint32_t i = base + tid;
@@ -324,7 +324,7 @@ __device__ __forceinline__ void expand_relax(const uint32_t* list, uint32_t coun
         __syncthreads();
         for (uint32_t k0 = tid; k0 < total; k0 += U * BS) {
             uint32_t pv[U], pe[U], pt[U];
-            B pdv[U], pcur[U];
+            B pdv[U];
             L pl[U];
             bool pok[U], pfin[U];
 #pragma unroll
@@ -337,7 +337,6 @@ __device__ __forceinline__ void expand_relax(const uint32_t* list, uint32_t coun
                 pt[j] = 0;
                 pl[j] = L(0);
                 pfin[j] = true;
-                pcur[j] = 0;
                 if (pok[j]) {
                     const uint32_t q = k >> cs;
                     uint32_t lo = S.t_own[q], hi = static_cast<uint32_t>(S.t_own[q + 1]) + 1u;
@@ -359,13 +358,13 @@ __device__ __forceinline__ void expand_relax(const uint32_t* list, uint32_t coun
             }
 #pragma unroll
             for (int j = 0; j < U; ++j) {
-                if (pok[j]) probe(pt[j], pl[j], pdv[j], pfin[j], pcur[j]);
+                if (pok[j]) pfin[j] = filter(pt[j], pl[j], pdv[j]);
             }
             B pold[U];
 #pragma unroll
-            for (int j = 0; j < U; ++j) pold[j] = lower(pok[j], pt[j], pdv[j], pl[j], pfin[j], pcur[j]);
+            for (int j = 0; j < U; ++j) pold[j] = update(pok[j], pt[j], pdv[j], pl[j], pfin[j]);
 #pragma unroll
-            for (int j = 0; j < U; ++j) apply(pok[j], pv[j], pdv[j], pe[j], pt[j], pl[j], pfin[j], pcur[j], pold[j]);
+            for (int j = 0; j < U; ++j) apply(pok[j], pv[j], pdv[j], pe[j], pt[j], pl[j], pfin[j], pold[j]);
         }
         __syncthreads();
     }
@@ -375,7 +374,7 @@ __device__ __forceinline__ void expand_relax(const uint32_t* list, uint32_t coun
 // per node and scale-free. A stored bound is never below the node's current distance: distances only fall, and
 // each improvement stores the bound of its own value, so a late store of an earlier (larger) value is still an
 // upper bound. A candidate above a node's bound can neither improve its distance nor be tight, so its global
-// distance probe is skipped. kBoundInf (+inf) is the bound of an unreached node.
+// distance update is skipped. kBoundInf (+inf) is the bound of an unreached node.
 constexpr uint16_t kBoundInf = 0x7F80u;
 
 template <class T>
@@ -392,7 +391,7 @@ __device__ __forceinline__ uint16_t bound16(typename T::B b) {
 }
 
 // Candidate c is above bound bd. Comparing in double is exact for every bound; a candidate that rounds onto
-// the bound is not above it, which only costs a probe.
+// the bound is not above it, which only costs an update.
 template <class T>
 __device__ __forceinline__ bool above_bound(typename T::B c, uint16_t bd) {
     return T::val(c) > static_cast<double>(__uint_as_float(static_cast<uint32_t>(bd) << 16));
@@ -409,15 +408,17 @@ __device__ __forceinline__ uint32_t code4(double x) { return x < 15.0 ? static_c
 // Arc loop of the relaxation passes (settle passes and frontier phases) in wide workgroups, where few arcs
 // survive the LDS filter (bitmap and bounds). Stage A: UA arcs per lane get their owner, CSR entry and
 // candidate, and are filtered. The survivors are appended to the wave's LDS queue (ranked slot by slot from
-// ballots alone), and stage B (distance probe, update, apply) runs only when the next survivors would not fit
-// and once at the end of the tile, one queued survivor per lane: the settle passes keep a few survivors per
-// iteration, so their probes and updates wait for memory once every few iterations instead of every time. More
-// survivors than a queue holds run stage B slot by slot as in expand_relax. The loop runs wave-uniform
-// iterations, so every lane takes part in each stage B (apply uses ballots).
-template <int UA, int BS, class B, class L, class NodeFn, class AddFn, class SkipFn, class ProbeFn, class LowerFn, class ApplyFn>
+// ballots alone), and stage B (update, apply) runs only when the next survivors would not fit and once at the
+// end of the tile, one queued survivor per lane: the settle passes keep a few survivors per iteration, so their
+// updates wait for memory once every few iterations instead of every time. More survivors than a queue holds
+// run stage B slot by slot. The loop runs wave-uniform iterations, so every lane takes part in each stage B
+// (apply uses ballots). Stage B issues the atomic minimum of every survivor at once, without a distance probe
+// first: its old value is the probe (the candidates that survive the filter mostly improve their target, so a
+// probe would save few atomics and cost a memory round trip).
+template <int UA, int BS, class B, class L, class NodeFn, class AddFn, class SkipFn, class UpdateFn, class ApplyFn>
 __device__ __forceinline__ void expand_relax_compact(const uint32_t* list, uint32_t count, Lds<B, BS>& S, gptr<uint32_t> tgt,
-                                                     gptr<L> len, NodeFn node_fn, AddFn add, SkipFn skip, ProbeFn probe,
-                                                     LowerFn lower, ApplyFn apply) {
+                                                     gptr<L> len, NodeFn node_fn, AddFn add, SkipFn skip, UpdateFn update,
+                                                     ApplyFn apply) {
     static_assert(BS >= 512, "the survivor queues are laid out for wide workgroups");
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = lane_id();
@@ -438,9 +439,8 @@ __device__ __forceinline__ void expand_relax_compact(const uint32_t* list, uint3
             cnd = qn[lane];
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");  // every lane has read before the queue is refilled
-        const B cur = live ? probe(ct) : B(0);
-        const B old = lower(live, ct, cnd, cur);
-        apply(live, ce, ct, cnd, cur, old);
+        const B old = update(live, ct, cnd);
+        apply(live, ce, ct, cnd, old);
         qc = 0;
     };
     for (uint32_t base = 0; base < count; base += BS) {
@@ -531,13 +531,11 @@ __device__ __forceinline__ void expand_relax_compact(const uint32_t* list, uint3
                 }
                 qc += nsv;
             } else {
-                B cur[UA], old[UA];
+                B old[UA];
 #pragma unroll
-                for (int j = 0; j < UA; ++j) cur[j] = sv[j] ? probe(st[j]) : B(0);
+                for (int j = 0; j < UA; ++j) old[j] = update(sv[j], st[j], snd[j]);
 #pragma unroll
-                for (int j = 0; j < UA; ++j) old[j] = lower(sv[j], st[j], snd[j], cur[j]);
-#pragma unroll
-                for (int j = 0; j < UA; ++j) apply(sv[j], se[j], st[j], snd[j], cur[j], old[j]);
+                for (int j = 0; j < UA; ++j) apply(sv[j], se[j], st[j], snd[j], old[j]);
             }
         }
         if (qc != 0u) drain();
@@ -671,7 +669,7 @@ __device__ __forceinline__ void for_row_words(const uint32_t* bits, uint32_t b, 
 #ifndef NG_SETTLE_ARCS_8B
 #define NG_SETTLE_ARCS_8B 3
 #endif
-// LD: the current source's distances live in LDS (graphs small enough; 256-lane workgroups), so probes and
+// LD: the current source's distances live in LDS (graphs small enough; 256-lane workgroups), so reads and
 // updates are LDS operations; otherwise they live in the slot's global array.
 template <class D, int MODE, int BS, bool LD>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? NG_WAVES_PER_EU : 4))) void batch_kernel(
@@ -683,7 +681,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
     constexpr uint32_t kW = BS / 64;
     // Bitmap (dynamic LDS) of the nodes registered in the settled list of the current source (this bucket or
     // an earlier one). It replaces the global settled stamps, and in a settle pass a heavy arc (length >=
-    // delta) into a registered node skips its global distance probe: the target's distance is final and
+    // delta) into a registered node skips its global distance update: the target's distance is final and
     // below hi while the candidate is at least hi, so the arc can neither improve nor be tight.
     // (16-byte aligned: the LDS distances start at an even word offset and are read as 8-byte values)
     extern __shared__ __attribute__((aligned(16))) uint32_t bm[];
@@ -879,25 +877,26 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
         };
         bsync();
 
-        // distance probe of arc target t: skipped (fin) for heavy arcs into registered nodes, and for candidates
-        // above the target's bound
-        auto probe = [&](uint32_t t, L l, B dv, bool& fin, B& cur) {
-            fin = use_bm && skip_ok && !(l < delta) && ((bm[t >> 5] >> (t & 31)) & 1u);
+        // filter of arc (v, t) with length l from v at distance dv (fin: skipped): a heavy arc into a registered
+        // node, or a candidate above the target's bound. The arcs that pass take part in the distance update.
+        auto filter = [&](uint32_t t, L l, B dv) -> bool {
+            bool fin = use_bm && skip_ok && !(l < delta) && ((bm[t >> 5] >> (t & 31)) & 1u);
             if (!fin) fin = bd_skip(t, T::add(dv, l));
-            cur = fin ? B(0) : ldc(DP(t));
             c_probe += fin ? 0u : 1u;
+            return fin;
         };
-        // distance update of arc (v, t) with length l from v at distance dv, given the probe (fin, cur): t's
-        // previous distance, or the candidate itself when no update was needed
-        auto lower = [&](bool ok, uint32_t t, B dv, L l, bool final_t, B cur) -> B {
+        // distance update of arc (v, t) with length l from v at distance dv that passed the filter: the atomic
+        // minimum, whose old value (t's previous distance) also serves as the probe; the candidate itself for a
+        // padding slot or a filtered arc
+        auto update = [&](bool ok, uint32_t t, B dv, L l, bool final_t) -> B {
             const B nd = T::add(dv, l);
-            return (ok && !final_t && nd < cur) ? wg_min(DP(t), nd) : nd;
+            return (ok && !final_t) ? wg_min(DP(t), nd) : nd;
         };
-        // relaxation of arc e into t with candidate distance nd, given the probe (live: probed; cur) and the
-        // update (old); new frontier entries are stamped `nph`. In the betweenness modes every candidate that
-        // could be tight (nd <= cur) is logged: light arcs by the frontier phases (a node is expanded again after
-        // each improvement, so also with its final distance), heavy arcs by the settle passes.
-        auto apply_core = [&](bool live, uint32_t e, uint32_t t, B nd, B cur, B old, uint32_t nph) {
+        // relaxation of arc e into t with candidate distance nd, given the update (live: updated; old: t's
+        // distance before it); new frontier entries are stamped `nph`. In the betweenness modes every candidate
+        // that could be tight (nd <= old) is logged: light arcs by the frontier phases (a node is expanded again
+        // after each improvement, so also with its final distance), heavy arcs by the settle passes.
+        auto apply_core = [&](bool live, uint32_t e, uint32_t t, B nd, B old, uint32_t nph) {
             bool pf = false, pp = false, parked = false;
             if (nd < old) {
                 ++c_improve;
@@ -927,7 +926,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                 if (lane_id() == 0) atomicMin(&S.runmin, m);
             }
             if constexpr (MODE != MODE_SSSP) {
-                const bool lg = live && nd <= cur;
+                const bool lg = live && nd <= old;
                 const unsigned long long mask = __ballot(lg);
                 if (mask != 0ull) {
                     const uint32_t lane = lane_id();
@@ -937,20 +936,21 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                 }
             }
         };
-        auto apply = [&](bool ok, uint32_t, B dv, uint32_t e, uint32_t t, L l, bool final_t, B cur, B old, uint32_t nph) {
-            apply_core(ok && !final_t, e, t, T::add(dv, l), cur, old, nph);
+        auto apply = [&](bool ok, uint32_t, B dv, uint32_t e, uint32_t t, L l, bool final_t, B old, uint32_t nph) {
+            apply_core(ok && !final_t, e, t, T::add(dv, l), old, nph);
         };
-        // the same steps for the compacted settle pass, given the candidate: the filter (heavy arc into a
-        // registered node, or candidate above the target's bound), the probe and the update
+        // the same steps for the compacted arc loop, given the candidate: the filter (heavy arc into a registered
+        // node, or candidate above the target's bound) and the update
         auto skip_arc = [&](uint32_t t, L l, B nd) -> bool {
             if (use_bm && skip_ok && !(l < delta) && ((bm[t >> 5] >> (t & 31)) & 1u)) return true;
             return bd_skip(t, nd);
         };
-        auto probe_t = [&](uint32_t t) -> B {
+        // the update of a survivor: t's previous distance (the candidate itself for a lane with no survivor)
+        auto update_nd = [&](bool live, uint32_t t, B nd) -> B {
+            if (!live) return nd;
             ++c_probe;
-            return ldc(DP(t));
+            return wg_min(DP(t), nd);
         };
-        auto lower_nd = [&](bool live, uint32_t t, B nd, B cur) -> B { return (live && nd < cur) ? wg_min(DP(t), nd) : nd; };
 
         while (true) {
             // ---- inner phases: light arcs of the frontier until the bucket [lo, hi) is settled
@@ -978,14 +978,13 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                 };
                 if constexpr (BS >= 512) {
                     expand_relax_compact<NG_PHASE_ARCS, BS>(
-                        Fc, nF, S, tgt, len, light_row, [](B dv, L l) -> B { return T::add(dv, l); }, skip_arc, probe_t,
-                        lower_nd,
-                        [&](bool live, uint32_t e, uint32_t t, B nd, B cur, B old) { apply_core(live, e, t, nd, cur, old, nph); });
+                        Fc, nF, S, tgt, len, light_row, [](B dv, L l) -> B { return T::add(dv, l); }, skip_arc, update_nd,
+                        [&](bool live, uint32_t e, uint32_t t, B nd, B old) { apply_core(live, e, t, nd, old, nph); });
                 } else {
                     expand_relax<NG_UNROLL, BS>(
-                        Fc, nF, S, tgt, len, light_row, probe, lower,
-                        [&](bool ok, uint32_t v, B dv, uint32_t e, uint32_t t, L l, bool fin, B cur, B old) {
-                            apply(ok, v, dv, e, t, l, fin, cur, old, nph);
+                        Fc, nF, S, tgt, len, light_row, filter, update,
+                        [&](bool ok, uint32_t v, B dv, uint32_t e, uint32_t t, L l, bool fin, B old) {
+                            apply(ok, v, dv, e, t, l, fin, old, nph);
                         });
                 }
                 // first visit in this bucket -> settled list
@@ -1060,13 +1059,13 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                     constexpr int kFilterArcs = sizeof(B) == 8 ? NG_SETTLE_ARCS_8B : NG_SETTLE_ARCS_4B;
                     expand_relax_compact<kFilterArcs, BS>(
                         Sl + settle_from, nS - settle_from, S, tgt, len, settle_row,
-                        [](B dv, L l) -> B { return T::add(dv, l); }, skip_arc, probe_t, lower_nd,
-                        [&](bool live, uint32_t e, uint32_t t, B nd, B cur, B old) { apply_core(live, e, t, nd, cur, old, nph); });
+                        [](B dv, L l) -> B { return T::add(dv, l); }, skip_arc, update_nd,
+                        [&](bool live, uint32_t e, uint32_t t, B nd, B old) { apply_core(live, e, t, nd, old, nph); });
                 } else {
                     expand_relax<NG_UNROLL, BS>(
-                        Sl + settle_from, nS - settle_from, S, tgt, len, settle_row, probe, lower,
-                        [&](bool ok, uint32_t v, B dv, uint32_t e, uint32_t t, L l, bool fin, B cur, B old) {
-                            apply(ok, v, dv, e, t, l, fin, cur, old, nph);
+                        Sl + settle_from, nS - settle_from, S, tgt, len, settle_row, filter, update,
+                        [&](bool ok, uint32_t v, B dv, uint32_t e, uint32_t t, L l, bool fin, B old) {
+                            apply(ok, v, dv, e, t, l, fin, old, nph);
                         });
                 }
                 bsync();
