@@ -83,7 +83,7 @@ def read_traffic(workload: str):
     if not os.path.exists(prof):
         return None, None
     with open(prof) as f:
-        return json.load(f).get("hbm_bytes_per_launch"), rel
+        return json.load(f).get("fabric_bytes_per_launch"), rel
 
 
 def roofline(achieved_gbs: float, workload: str) -> dict:

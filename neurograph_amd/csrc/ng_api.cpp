@@ -144,7 +144,7 @@ struct ng_cohort {
     uint32_t slots = 0;
     uint64_t ws_generation = 0;        // context workspace generation the capture was made against
     CohortGraph* d_gtab = nullptr;
-    uint32_t* d_lend = nullptr;        // light-prefix ends of every member for its own width
+    RowRange* d_rows = nullptr;        // row bounds and light-prefix ends of every member for its own width
     uint32_t* d_prefix = nullptr;
     uint64_t* d_fbase = nullptr;
     uint32_t* d_seg_ptr = nullptr;
@@ -301,7 +301,7 @@ ng_status build_view_typed(ng_graph* g, int vi, DKind dk) {
         NG_HIP(hipMalloc(&v.tgt, sizeof(uint32_t) * std::max<uint64_t>(m, 1)));
         NG_HIP(hipMalloc(&v.len, sizeof(L) * std::max<uint64_t>(m, 1)));
         NG_HIP(hipMalloc(&v.perm, sizeof(uint32_t) * std::max<uint64_t>(m, 1)));
-        NG_HIP(hipMalloc(&v.lend, sizeof(uint32_t) * std::max<uint32_t>(n, 1)));
+        NG_HIP(hipMalloc(&v.rows, sizeof(RowRange) * std::max<uint32_t>(n, 1)));
         NG_HIP(hipMemcpy(v.off, off32.data(), sizeof(uint32_t) * off32.size(), hipMemcpyHostToDevice));
     } else {
         // earlier calls on the context stream may still read the arrays
@@ -364,7 +364,7 @@ void free_view(View& v) {
     if (v.len) (void)hipFree(v.len);
     v.len = nullptr;
     dfree(v.perm);
-    dfree(v.lend);
+    dfree(v.rows);
     std::vector<uint32_t>().swap(v.perm_host);
     v.built = false;
 }
@@ -590,7 +590,7 @@ ng_status run_batch(ng_graph* g, const RunSpec& rs, uint32_t* slots_used, double
     const double delta = pick_delta(g, v, rs.vi, mode_idx, rs.delta);
     if (used_delta) *used_delta = delta;
     if (v.lend_delta != delta) {
-        NG_HIP(launch_light_end(v.dk, g->n, v.off, v.len, delta, v.lend, ctx->stream));
+        NG_HIP(launch_light_end(v.dk, g->n, v.off, v.len, delta, v.rows, ctx->stream));
         v.lend_delta = delta;
     }
     // slots: resident workgroups of this kernel, bounded by the sources and by device memory
@@ -1150,7 +1150,7 @@ ng_status ng_brandes_accumulate_noalloc(ng_graph* g, const uint32_t* sources, ui
     const hipStream_t s = ctx->stream;
     const double d = pick_delta(g, v, 0, 1, delta);
     if (v.lend_delta != d) {
-        NG_HIP(launch_light_end(v.dk, n, v.off, v.len, d, v.lend, s));
+        NG_HIP(launch_light_end(v.dk, n, v.off, v.len, d, v.rows, s));
         v.lend_delta = d;
     }
     const int block = pick_block(v, 1, 0, g->opt_block);
@@ -1410,7 +1410,7 @@ void cohort_free_device(ng_cohort* c) {
     c->exec = nullptr;
     c->graph = nullptr;
     dfree(c->d_gtab);
-    dfree(c->d_lend);
+    dfree(c->d_rows);
     dfree(c->d_prefix);
     dfree(c->d_fbase);
     dfree(c->d_seg_ptr);
@@ -1467,12 +1467,12 @@ ng_status cohort_build(ng_cohort* c) {
     c->versions.resize(G);
     for (uint32_t gi = 0; gi < G; ++gi) c->versions[gi] = c->graphs[gi]->weights_version;
     // per-graph light-prefix ends for the cohort's own widths
-    NG_HIP(hipMalloc(&c->d_lend, 4 * std::max<size_t>(c->total, 1)));
+    NG_HIP(hipMalloc(&c->d_rows, sizeof(RowRange) * std::max<size_t>(c->total, 1)));
     for (uint32_t gi = 0; gi < G; ++gi) {
         const View& v = c->graphs[gi]->views[0];
-        uint32_t* lend = c->d_lend + c->prefix[gi];
-        NG_HIP(launch_light_end(v.dk, c->graphs[gi]->n, v.off, v.len, eff_delta[gi], lend, st));
-        gtab[gi].lend = lend;
+        RowRange* rows = c->d_rows + c->prefix[gi];
+        NG_HIP(launch_light_end(v.dk, c->graphs[gi]->n, v.off, v.len, eff_delta[gi], rows, st));
+        gtab[gi].rows = rows;
     }
     // slots
     const int maps = lds_maps_for(c->dk, MODE_BC, c->n_max, c->block, false);

@@ -46,6 +46,16 @@ enum class DKind : int { U32 = 0, U64 = 1, F32 = 2, F64 = 3 };
 // Kernel modes (bit flags). SSSP = distances + summaries only.
 enum : int { MODE_SSSP = 0, MODE_BC = 1, MODE_ARC1 = 2, MODE_ARC2 = 4 };
 
+// Bounds of row v in one 16-byte record: its first arc, the end of its light prefix (arcs shorter than the
+// bucket width) and its end. A relaxation pass touches a node's record with one load instead of two or three
+// loads from separate offset arrays, each of which would fetch a line of its own.
+struct alignas(16) RowRange {
+    uint32_t beg;
+    uint32_t light_end;
+    uint32_t end;
+    uint32_t pad;
+};
+
 // Device CSR of one length kind. Rows are sorted by (length, target) so that the arcs lighter than the
 // bucket width form a prefix of each row; perm maps a sorted position back to the caller's arc index.
 struct View {
@@ -61,7 +71,7 @@ struct View {
     int tuned_block[2] = {0, 0};         // lanes per workgroup per mode; 0 = not tuned
     double lend_delta = -1.0;
     double bound_inv_q = 0.0;  // scale of the 4-bit distance codes: 15 / (1.1 x largest sampled distance); 0 = not estimated
-    uint32_t* lend = nullptr;  // [n] end of the light prefix of each row for lend_delta
+    RowRange* rows = nullptr;  // [n] row bounds of each node with the end of its light prefix for lend_delta
     std::vector<uint32_t> perm_host;  // host copy of perm: weight updates re-sort only the rows whose order changes
     bool reordered_last = false;      // the last build or weight update changed the row order
 };
@@ -85,6 +95,7 @@ struct LogEntry {
     uint32_t e;
     unsigned long long nd;
 };
+static_assert(sizeof(LogEntry) == 16, "log entries are written and read as one 16-byte vector");
 
 // Per-slot scratch of the batched kernel, one per context, shared by its graphs and cohorts (calls on a
 // context are sequential). Sized for (slots, n, m) = the largest request so far; grown, never shrunk. A
@@ -138,7 +149,7 @@ struct CohortGraph {
     const uint32_t* off;
     const uint32_t* tgt;
     const void* len;
-    const uint32_t* lend;
+    const RowRange* rows;
     const uint32_t* perm;
     uint32_t n;
     uint32_t pad;
@@ -170,7 +181,7 @@ struct LaunchArgs {
 
 // Launch wrappers (ng_kernels.hip). All enqueue on `stream`.
 hipError_t launch_batch(const LaunchArgs& a, hipStream_t stream);
-hipError_t launch_light_end(DKind dk, uint32_t n, const uint32_t* off, const void* len, double delta, uint32_t* lend,
+hipError_t launch_light_end(DKind dk, uint32_t n, const uint32_t* off, const void* len, double delta, RowRange* rows,
                             hipStream_t stream);
 hipError_t launch_reduce_slots(const BcRec* rec, uint32_t slots, uint32_t n, uint32_t stride, double* out, hipStream_t stream);
 // Sets every record to the clean state (all zero).

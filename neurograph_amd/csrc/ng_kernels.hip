@@ -140,6 +140,9 @@ template <class T>
 __device__ __forceinline__ gptr<T> as_global(const T* p) {
     return (gptr<T>)p;
 }
+// A node's row record (RowRange: first arc, light-prefix end, end) as one 16-byte load.
+using u32x4 = uint32_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ u32x4 load_row(gptr<RowRange> rows, uint32_t v) { return ((gptr<u32x4>)rows)[v]; }
 // Phase boundary: a workgroup barrier with workgroup-scope release/acquire.
 __device__ __forceinline__ void bsync() { __syncthreads(); }
 
@@ -288,13 +291,15 @@ __device__ __forceinline__ void expand(const uint32_t* list, uint32_t count, Lds
 // outstanding at every step instead of one. The owner of arc k lies between the
 // owners of the first arcs of its chunk and of the next chunk (a per-tile table of at most 2 BS chunks), so
 // the search is short instead of log2(BS) dependent LDS reads. apply() runs on every active lane (it may
-// append to queues with wave ballots), with ok = false for padding slots.
+// append to queues with wave ballots), with ok = false for padding slots. tile_fn(valid, dv) runs on every lane
+// once the tile's nodes are loaded (valid: the lane holds a node, at distance dv).
 #ifndef NG_UNROLL
 #define NG_UNROLL 2
 #endif
-template <int U, int BS, class B, class L, class NodeFn, class FilterFn, class UpdateFn, class ApplyFn>
+template <int U, int BS, class B, class L, class NodeFn, class TileFn, class FilterFn, class UpdateFn, class ApplyFn>
 __device__ __forceinline__ void expand_relax(const uint32_t* list, uint32_t count, Lds<B, BS>& S, gptr<uint32_t> tgt,
-                                             gptr<L> len, NodeFn node_fn, FilterFn filter, UpdateFn update, ApplyFn apply) {
+                                             gptr<L> len, NodeFn node_fn, TileFn tile_fn, FilterFn filter, UpdateFn update,
+                                             ApplyFn apply) {
     const uint32_t tid = threadIdx.x;
     for (uint32_t base = 0; base < count; base += BS) {
         const uint32_t i = base + tid;
@@ -304,6 +309,7 @@ __device__ __forceinline__ void expand_relax(const uint32_t* list, uint32_t coun
             v = list[i];
             node_fn(v, dv, beg, cnt);
         }
+        tile_fn(i < count, dv);
         uint32_t total;
         const uint32_t ex = block_excl_scan<BS>(cnt, &total, S.wave);
         S.t_v[tid] = v;
@@ -414,11 +420,11 @@ __device__ __forceinline__ uint32_t code4(double x) { return x < 15.0 ? static_c
 // run stage B slot by slot. The loop runs wave-uniform iterations, so every lane takes part in each stage B
 // (apply uses ballots). Stage B issues the atomic minimum of every survivor at once, without a distance probe
 // first: its old value is the probe (the candidates that survive the filter mostly improve their target, so a
-// probe would save few atomics and cost a memory round trip).
-template <int UA, int BS, class B, class L, class NodeFn, class AddFn, class SkipFn, class UpdateFn, class ApplyFn>
+// probe would save few atomics and cost a memory round trip). tile_fn as in expand_relax.
+template <int UA, int BS, class B, class L, class NodeFn, class TileFn, class AddFn, class SkipFn, class UpdateFn, class ApplyFn>
 __device__ __forceinline__ void expand_relax_compact(const uint32_t* list, uint32_t count, Lds<B, BS>& S, gptr<uint32_t> tgt,
-                                                     gptr<L> len, NodeFn node_fn, AddFn add, SkipFn skip, UpdateFn update,
-                                                     ApplyFn apply) {
+                                                     gptr<L> len, NodeFn node_fn, TileFn tile_fn, AddFn add, SkipFn skip,
+                                                     UpdateFn update, ApplyFn apply) {
     static_assert(BS >= 512, "the survivor queues are laid out for wide workgroups");
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = lane_id();
@@ -451,6 +457,7 @@ __device__ __forceinline__ void expand_relax_compact(const uint32_t* list, uint3
             v = list[i];
             node_fn(v, dv, beg, cnt);
         }
+        tile_fn(i < count, dv);
         uint32_t total;
         const uint32_t ex = block_excl_scan<BS>(cnt, &total, S.wave);
         S.t_v[tid] = v;
@@ -551,7 +558,7 @@ struct KArgs {
     const uint32_t* off;
     const uint32_t* tgt;
     const L* len;
-    const uint32_t* lend;
+    const RowRange* rows;
     const uint32_t* perm;
     const uint32_t* sources;
     uint32_t n_sources;
@@ -739,7 +746,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
     gptr<uint32_t> off = as_global(a.off);
     gptr<uint32_t> tgt = as_global(a.tgt);
     gptr<L> len = as_global(a.len);
-    gptr<uint32_t> lend = as_global(a.lend);
+    gptr<RowRange> rows = as_global(a.rows);
     gptr<uint32_t> perm = as_global(a.perm);
     uint32_t bmw = a.bm_words;  // bitmap words of the current graph
     B* const dist = LD ? reinterpret_cast<B*>(bm + a.ld_off) : a.dist + so;
@@ -806,7 +813,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                 off = as_global(cg.off);
                 tgt = as_global(cg.tgt);
                 len = as_global(static_cast<const L*>(cg.len));
-                lend = as_global(cg.lend);
+                rows = as_global(cg.rows);
                 perm = as_global(cg.perm);
                 delta = static_cast<L>(cg.delta);
                 bmw = use_bm ? (n + 31) / 32 : 0u;
@@ -841,6 +848,10 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
             const uint32_t bdw = bd4 ? (n + 7) / 8 : (n + 1) / 2;
             const uint32_t fillv = bd4 ? 0xFFFFFFFFu : (static_cast<uint32_t>(kBoundInf) << 16) | kBoundInf;
             for (uint32_t w = tid; w < bdw; w += BS) flw[w] = fillv;
+        }
+        if (tid < 3 * kW) {  // the waves' summary sums (the previous source's were read before its reset)
+            S.red_u[tid] = 0;
+            S.red_d[tid] = 0.0;
         }
         bsync();
         if (tid == 0) {
@@ -932,7 +943,17 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                     const uint32_t lane = lane_id();
                     const uint32_t rank = static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
                     const uint32_t base = wave_reserve(&S.cnt[3], static_cast<uint32_t>(__popcll(mask)));
-                    if (lg && base + rank < a.logcap) log[base + rank] = LogEntry{t, e, static_cast<unsigned long long>(nd)};
+                    // the log is written once here and read once by the verification: non-temporal stores (and
+                    // loads there), so its lines leave the caches first and the distances and rows stay longer
+                    if (lg && base + rank < a.logcap) {
+                        const unsigned long long ndu = static_cast<unsigned long long>(nd);
+                        u32x4 q;  // LogEntry{t, e, nd} as one 16-byte store
+                        q.x = t;
+                        q.y = e;
+                        q.z = static_cast<uint32_t>(ndu);
+                        q.w = static_cast<uint32_t>(ndu >> 32);
+                        __builtin_nontemporal_store(q, reinterpret_cast<u32x4*>(log + base + rank));
+                    }
                 }
             }
         };
@@ -972,17 +993,18 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                 }
                 auto light_row = [&](uint32_t v, B& dv, uint32_t& beg, uint32_t& cnt) {
                     dv = ldc(DP(v));
-                    beg = off[v];
-                    cnt = lend[v] - beg;
+                    const u32x4 r = load_row(rows, v);  // x: first arc, y: light-prefix end
+                    beg = r.x;
+                    cnt = r.y - r.x;
                     c_light += cnt;
                 };
                 if constexpr (BS >= 512) {
                     expand_relax_compact<NG_PHASE_ARCS, BS>(
-                        Fc, nF, S, tgt, len, light_row, [](B dv, L l) -> B { return T::add(dv, l); }, skip_arc, update_nd,
+                        Fc, nF, S, tgt, len, light_row, [](bool, B) {}, [](B dv, L l) -> B { return T::add(dv, l); }, skip_arc, update_nd,
                         [&](bool live, uint32_t e, uint32_t t, B nd, B old) { apply_core(live, e, t, nd, old, nph); });
                 } else {
                     expand_relax<NG_UNROLL, BS>(
-                        Fc, nF, S, tgt, len, light_row, filter, update,
+                        Fc, nF, S, tgt, len, light_row, [](bool, B) {}, filter, update,
                         [&](bool ok, uint32_t v, B dv, uint32_t e, uint32_t t, L l, bool fin, B old) {
                             apply(ok, v, dv, e, t, l, fin, old, nph);
                         });
@@ -1046,11 +1068,45 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                 NG_TICK(t_settle);
                 const uint32_t nph = phase + 1;
                 // heavy arcs only: the light ones were relaxed, and their candidates logged, by the phases
+                uint32_t deg = 0;  // out-degree of the lane's node
                 auto settle_row = [&](uint32_t v, B& dv, uint32_t& beg, uint32_t& cnt) {
                     dv = ldc(DP(v));
-                    beg = lend[v];
-                    cnt = off[v + 1] - beg;
+                    const u32x4 r = load_row(rows, v);  // x: first arc, y: light-prefix end, z: end
+                    beg = r.y;
+                    cnt = r.z - beg;
                     c_settle += cnt;
+                    deg = r.z - r.x;
+                };
+                // Every reached node is settled exactly once, with its final distance: the per-source summary is
+                // accumulated per tile (wave sums added to the wave's summary slots) instead of in a pass of its own
+                // over the distances in global memory. With LDS distances that pass is cheap and many small tiles
+                // would each pay the reductions, so the summary keeps its own pass.
+                auto settle_tile = [&](bool valid, B dv) {
+                    if constexpr (LD) return;
+                    unsigned long long su = 0, mx = 0, arcs = valid ? deg : 0u;
+                    double sd = 0.0, se = 0.0;
+                    if (valid) {
+                        if constexpr (T::kInt) su = static_cast<unsigned long long>(dv);
+                        else sd = T::val(dv);
+                        if (dv != B(0)) se = 1.0 / T::val(dv);
+                        mx = static_cast<unsigned long long>(dv);
+                    }
+                    auto addu = [](unsigned long long x, unsigned long long y) { return x + y; };
+                    auto maxu = [](unsigned long long x, unsigned long long y) { return x > y ? x : y; };
+                    auto addd = [](double x, double y) { return x + y; };
+                    if constexpr (T::kInt) su = wave_reduce(su, addu);
+                    else sd = wave_reduce(sd, addd);
+                    se = wave_reduce(se, addd);
+                    mx = wave_reduce(mx, maxu);
+                    arcs = wave_reduce(arcs, addu);
+                    const uint32_t w = tid >> 6;
+                    if (lane_id() == 0) {
+                        S.red_u[w] += su;
+                        S.red_u[kW + w] = mx > S.red_u[kW + w] ? mx : S.red_u[kW + w];
+                        S.red_u[2 * kW + w] += arcs;
+                        S.red_d[w] += sd;
+                        S.red_d[kW + w] += se;
+                    }
                 };
                 if constexpr (BS >= 512) {
                     // wide workgroups: compacted survivors (the 256-lane kernel's register budget is too small)
@@ -1058,12 +1114,12 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                     // +1 % on spatial50k), more for 4-byte ones (6 vs 4: +0.6 % on spatial200k)
                     constexpr int kFilterArcs = sizeof(B) == 8 ? NG_SETTLE_ARCS_8B : NG_SETTLE_ARCS_4B;
                     expand_relax_compact<kFilterArcs, BS>(
-                        Sl + settle_from, nS - settle_from, S, tgt, len, settle_row,
+                        Sl + settle_from, nS - settle_from, S, tgt, len, settle_row, settle_tile,
                         [](B dv, L l) -> B { return T::add(dv, l); }, skip_arc, update_nd,
                         [&](bool live, uint32_t e, uint32_t t, B nd, B old) { apply_core(live, e, t, nd, old, nph); });
                 } else {
                     expand_relax<NG_UNROLL, BS>(
-                        Sl + settle_from, nS - settle_from, S, tgt, len, settle_row, filter, update,
+                        Sl + settle_from, nS - settle_from, S, tgt, len, settle_row, settle_tile, filter, update,
                         [&](bool ok, uint32_t v, B dv, uint32_t e, uint32_t t, L l, bool fin, B old) {
                             apply(ok, v, dv, e, t, l, fin, old, nph);
                         });
@@ -1172,36 +1228,40 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
 
         NG_TICK(t_summary);
         const uint32_t nS = S.cnt[2] > n ? n : S.cnt[2];
-        // ---- per-source summary (SsspResult.h:43-64) over the reached nodes
+        // ---- per-source summary (SsspResult.h:43-64) over the reached nodes: the waves' sums of the settle passes,
+        // or with LDS distances a pass over the reached nodes
         {
-            unsigned long long su = 0, mx = 0, arcs = 0;
-            double sd = 0.0, se = 0.0;
-            for (uint32_t i = tid; i < nS; i += BS) {
-                const uint32_t v = Sl[i];
-                const B d = ldc(DP(v));
-                if constexpr (T::kInt) su += static_cast<unsigned long long>(d);
-                else sd += T::val(d);
-                if (d != B(0)) se += 1.0 / T::val(d);
-                mx = (static_cast<unsigned long long>(d) > mx) ? static_cast<unsigned long long>(d) : mx;
-                arcs += off[v + 1] - off[v];
+            if constexpr (LD) {
+                unsigned long long su = 0, mx = 0, arcs = 0;
+                double sd = 0.0, se = 0.0;
+                for (uint32_t i = tid; i < nS; i += BS) {
+                    const uint32_t v = Sl[i];
+                    const B d = ldc(DP(v));
+                    if constexpr (T::kInt) su += static_cast<unsigned long long>(d);
+                    else sd += T::val(d);
+                    if (d != B(0)) se += 1.0 / T::val(d);
+                    mx = (static_cast<unsigned long long>(d) > mx) ? static_cast<unsigned long long>(d) : mx;
+                    const u32x4 r = load_row(rows, v);
+                    arcs += r.z - r.x;
+                }
+                auto addu = [](unsigned long long x, unsigned long long y) { return x + y; };
+                auto maxu = [](unsigned long long x, unsigned long long y) { return x > y ? x : y; };
+                auto addd = [](double x, double y) { return x + y; };
+                su = wave_reduce(su, addu);
+                mx = wave_reduce(mx, maxu);
+                arcs = wave_reduce(arcs, addu);
+                sd = wave_reduce(sd, addd);
+                se = wave_reduce(se, addd);
+                const uint32_t w = tid >> 6;
+                if (lane_id() == 0) {
+                    S.red_u[w] = su;
+                    S.red_u[kW + w] = mx;
+                    S.red_u[2 * kW + w] = arcs;
+                    S.red_d[w] = sd;
+                    S.red_d[kW + w] = se;
+                }
+                __syncthreads();
             }
-            auto addu = [](unsigned long long x, unsigned long long y) { return x + y; };
-            auto maxu = [](unsigned long long x, unsigned long long y) { return x > y ? x : y; };
-            auto addd = [](double x, double y) { return x + y; };
-            su = wave_reduce(su, addu);
-            mx = wave_reduce(mx, maxu);
-            arcs = wave_reduce(arcs, addu);
-            sd = wave_reduce(sd, addd);
-            se = wave_reduce(se, addd);
-            const uint32_t w = tid >> 6;
-            if (lane_id() == 0) {
-                S.red_u[w] = su;
-                S.red_u[kW + w] = mx;
-                S.red_u[2 * kW + w] = arcs;
-                S.red_d[w] = sd;
-                S.red_d[kW + w] = se;
-            }
-            __syncthreads();
             if (tid == 0) {
                 unsigned long long tsu = 0, tmx = 0, tar = 0;
                 double tsd = 0.0, tse = 0.0;
@@ -1251,7 +1311,11 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
             auto each_candidate = [&](auto fn) {
                 if (nlog <= a.logcap) {
                     for (uint32_t i = tid; i < nlog; i += BS) {
-                        const LogEntry r = log[i];
+                        const u32x4 q = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(log + i));
+                        LogEntry r;
+                        r.t = q.x;
+                        r.e = q.y;
+                        r.nd = static_cast<unsigned long long>(q.z) | (static_cast<unsigned long long>(q.w) << 32);
                         fn(r.t, r.e, static_cast<B>(r.nd));
                     }
                 } else {
@@ -1324,8 +1388,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                             const uint32_t pos = atomicAdd(&S.cnt[4], k);
                             auto put = [&](uint32_t j, uint32_t t) {
                                 if (pos + j < n) {
-                                    order[pos + j] = t;
-                                    if (push_dep) par[pos + j] = u;
+                                    // order and parents are read once, by the dependency layers (non-temporal)
+                                    __builtin_nontemporal_store(t, order + pos + j);
+                                    if (push_dep) __builtin_nontemporal_store(u, par + pos + j);
                                 }
                             };
                             put(0, t0);
@@ -1393,8 +1458,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                 for (int ly = static_cast<int>(nl) - 1; ly >= 1; --ly) {
                     const uint32_t l0 = layers[ly], l1 = layers[ly + 1];
                     for (uint32_t i = l0 + tid; i < l1; i += BS) {
-                        const uint32_t t = order[i];
-                        const uint32_t p = par[i];
+                        const uint32_t t = __builtin_nontemporal_load(order + i);
+                        const uint32_t p = __builtin_nontemporal_load(par + i);
                         const double d = ldc(&R[t].delta1);
                         if (d != 0.0) {
                             R[t].bc += d;
@@ -1537,15 +1602,16 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
 
 // end of the light prefix of each (length-sorted) row
 template <class L>
-__global__ void light_end_kernel(uint32_t n, const uint32_t* off, const L* len, L delta, uint32_t* lend) {
+__global__ void light_end_kernel(uint32_t n, const uint32_t* off, const L* len, L delta, RowRange* rows) {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
-    uint32_t lo = off[v], hi = off[v + 1];
+    const uint32_t beg = off[v], end = off[v + 1];
+    uint32_t lo = beg, hi = end;
     while (lo < hi) {
         const uint32_t mid = lo + ((hi - lo) >> 1);
         if (len[mid] < delta) lo = mid + 1; else hi = mid;
     }
-    lend[v] = lo;
+    rows[v] = RowRange{beg, lo, end, 0u};
 }
 
 // out[v] = sum over slots of the betweenness partial of node v, in slot order (deterministic)
@@ -1658,7 +1724,7 @@ hipError_t launch_typed(const LaunchArgs& la, hipStream_t stream) {
         k.off = vw.off;
         k.tgt = vw.tgt;
         k.len = static_cast<const L*>(vw.len);
-        k.lend = vw.lend;
+        k.rows = vw.rows;
         k.perm = vw.perm;
     } else {
         k.gtab = la.gtab;
@@ -1791,29 +1857,29 @@ int batch_blocks_per_cu(DKind dk, int mode, uint32_t n, int maps, int block) {
     return 1;
 }
 
-hipError_t launch_light_end(DKind dk, uint32_t n, const uint32_t* off, const void* len, double delta, uint32_t* lend,
+hipError_t launch_light_end(DKind dk, uint32_t n, const uint32_t* off, const void* len, double delta, RowRange* rows,
                             hipStream_t stream) {
     if (n == 0) return hipSuccess;
     const dim3 grid((n + 255) / 256), block(256);
     switch (dk) {
     case DKind::U32: {
         const uint32_t d = static_cast<uint32_t>((delta < 1.0 ? 1.0 : delta) + 0.5);
-        hipLaunchKernelGGL(light_end_kernel<uint32_t>, grid, block, 0, stream, n, off, static_cast<const uint32_t*>(len), d, lend);
+        hipLaunchKernelGGL(light_end_kernel<uint32_t>, grid, block, 0, stream, n, off, static_cast<const uint32_t*>(len), d, rows);
         break;
     }
     case DKind::U64: {
         const unsigned long long d = static_cast<unsigned long long>((delta < 1.0 ? 1.0 : delta) + 0.5);
         hipLaunchKernelGGL(light_end_kernel<unsigned long long>, grid, block, 0, stream, n, off,
-                           static_cast<const unsigned long long*>(len), d, lend);
+                           static_cast<const unsigned long long*>(len), d, rows);
         break;
     }
     case DKind::F32:
         hipLaunchKernelGGL(light_end_kernel<float>, grid, block, 0, stream, n, off, static_cast<const float*>(len),
-                           static_cast<float>(delta), lend);
+                           static_cast<float>(delta), rows);
         break;
     case DKind::F64:
         hipLaunchKernelGGL(light_end_kernel<double>, grid, block, 0, stream, n, off, static_cast<const double*>(len), delta,
-                           lend);
+                           rows);
         break;
     }
     return hipGetLastError();
