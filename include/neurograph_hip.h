@@ -186,8 +186,9 @@ ng_status ng_brandes_accumulate(ng_graph* g, const uint32_t* sources, uint32_t n
  * memory (source slots, per-source summaries, the betweenness reduction) is one fixed device array of 256 MiB per
  * GPU, allocated when the library's code is loaded on that GPU and held until the process ends; its address is
  * resolved by ng_context_create. Graphs whose one source slot does not fit it are rejected with NG_EINVAL (a slot
- * takes about 170 bytes per node plus 16 bytes per arc up to 8 arcs per node: roughly 880 000 nodes at mean
- * out-degree 20, 2.7 million at mean out-degree 4; the 200 000-node spatial graph gets 4 slots); fewer slots
+ * takes about 165 bytes per node plus 16 bytes per arc up to 8 arcs per node, 20 with 8-byte distances: roughly
+ * 900 000 nodes at mean out-degree 20 and 1.15 million at mean out-degree 4 with 4-byte distances, 800 000 and
+ * 1.06 million with 8-byte ones; the 200 000-node spatial graph gets 4 slots, the 50 000-node one 16); fewer slots
  * fit than the GPU could run at once for large graphs, so it is slower there than ng_brandes_accumulate. Every call
  * re-initialises the array, which keeps the last call's contents afterwards; calls that overlap on one GPU (from
  * several threads or contexts) race on it and must be serialised by the caller. Sources are run in launches of

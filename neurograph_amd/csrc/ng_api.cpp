@@ -384,6 +384,7 @@ void free_ws(Workspace& w) {
     dfree(w.layers);
     dfree(w.bits);
     dfree(w.log);
+    dfree(w.log_hi);
     if (w.summaries) (void)hipFree(w.summaries);
     w.summaries = nullptr;
     dfree(w.arcs_reached);
@@ -446,7 +447,7 @@ uint64_t slot_bytes(uint32_t n, uint64_t m, int mode, uint32_t logcap) {
     uint64_t b = static_cast<uint64_t>(n) * (8 + 4 + 4 + 12) + 16 + 32ull * pile_capacity(n);
     if (mode != MODE_SSSP) {
         b += static_cast<uint64_t>(n) * (sizeof(BcRec) + 8 + 4 + 4) + 4 + ((m + 31) / 32) * 4 +
-             static_cast<uint64_t>(logcap) * sizeof(LogEntry);
+             static_cast<uint64_t>(logcap) * (sizeof(LogEntry) + 4);
     }
     return b;
 }
@@ -519,6 +520,7 @@ ng_status ensure_ws(Workspace& w, hipStream_t st, uint32_t n, uint64_t m, uint32
             NG_HIP(hipMalloc(&w.layers, 4 * static_cast<size_t>(slots) * (n + 1)));
             NG_HIP(hipMalloc(&w.bits, 4 * std::max<uint64_t>(w.words * slots, 1)));
             NG_HIP(hipMalloc(&w.log, sizeof(LogEntry) * static_cast<size_t>(w.logcap) * slots));
+            NG_HIP(hipMalloc(&w.log_hi, 4 * std::max<size_t>(static_cast<size_t>(w.logcap) * slots, 1)));
             NG_HIP(hipMalloc(&w.bc_out, 8 * std::max<uint32_t>(n, 1)));
             NG_HIP(hipMalloc(&w.grad1, 8 * std::max<uint64_t>(m, 1)));
             NG_HIP(hipMalloc(&w.grad2, 8 * std::max<uint64_t>(m, 1)));
@@ -1105,6 +1107,7 @@ size_t scratch_layout(char* base, Workspace& w, uint32_t n, uint64_t m, DKind dk
     w.layers = reinterpret_cast<uint32_t*>(take(4ull * slots * (static_cast<size_t>(n) + 1)));
     w.bits = reinterpret_cast<uint32_t*>(take(4ull * words * slots));
     w.log = reinterpret_cast<LogEntry*>(take(sizeof(LogEntry) * static_cast<size_t>(logcap) * slots));
+    if (dk_size(dk) == 8) w.log_hi = reinterpret_cast<uint32_t*>(take(4ull * logcap * slots));  // 8-byte distances only
     w.slots = slots;
     w.n = n;
     w.m = m;

@@ -89,11 +89,13 @@ struct alignas(32) BcRec {
 };
 static_assert(sizeof(BcRec) == 32, "BcRec must be a quarter of a 128-byte line");
 
-// A candidate tight arc logged by a settle pass: arc e into t with candidate distance nd (bits).
+// A candidate tight arc logged by the relaxation passes: arc e from u into t with candidate distance nd (bits).
+// The entry holds nd's low 32 bits; 8-byte distances keep their high 32 bits in a parallel array (log_hi).
 struct LogEntry {
     uint32_t t;
     uint32_t e;
-    unsigned long long nd;
+    uint32_t u;
+    uint32_t nd_lo;
 };
 static_assert(sizeof(LogEntry) == 16, "log entries are written and read as one 16-byte vector");
 
@@ -119,6 +121,7 @@ struct Workspace {
     uint32_t* bits = nullptr;   // [slots][words]
     uint64_t words = 0;
     LogEntry* log = nullptr;    // [slots][logcap]
+    uint32_t* log_hi = nullptr; // [slots][logcap] high halves of 8-byte candidate distances
     void* summaries = nullptr;  // [max_sources] ng_sssp_summary
     uint32_t summaries_cap = 0;
     uint64_t* arcs_reached = nullptr;  // [16]: arcs reached, phases, buckets, light arcs, settle arcs, logged, layers,

@@ -227,11 +227,12 @@ struct Lds {
     B runmin;
     unsigned long long red_u[3 * kW];
     double red_d[3 * kW];
-    // wide workgroups: one queue of 64 filter survivors (target, arc, candidate) per wave
+    // wide workgroups: one queue of 64 filter survivors (target, arc, candidate, tile slot of the arc's tail) per wave
     static constexpr uint32_t kQ = BS >= 512 ? BS : 1;
     uint32_t q_t[kQ];
     uint32_t q_e[kQ];
     B q_n[kQ];
+    uint16_t q_o[kQ];
 };
 
 template <int BS>
@@ -420,7 +421,8 @@ __device__ __forceinline__ uint32_t code4(double x) { return x < 15.0 ? static_c
 // run stage B slot by slot. The loop runs wave-uniform iterations, so every lane takes part in each stage B
 // (apply uses ballots). Stage B issues the atomic minimum of every survivor at once, without a distance probe
 // first: its old value is the probe (the candidates that survive the filter mostly improve their target, so a
-// probe would save few atomics and cost a memory round trip). tile_fn as in expand_relax.
+// probe would save few atomics and cost a memory round trip). tile_fn as in expand_relax; apply(live, e, t, nd, old, u)
+// also receives the arc's tail u.
 template <int UA, int BS, class B, class L, class NodeFn, class TileFn, class AddFn, class SkipFn, class UpdateFn, class ApplyFn>
 __device__ __forceinline__ void expand_relax_compact(const uint32_t* list, uint32_t count, Lds<B, BS>& S, gptr<uint32_t> tgt,
                                                      gptr<L> len, NodeFn node_fn, TileFn tile_fn, AddFn add, SkipFn skip,
@@ -432,21 +434,23 @@ __device__ __forceinline__ void expand_relax_compact(const uint32_t* list, uint3
     uint32_t* const qt = S.q_t + (tid & ~63u);
     uint32_t* const qe = S.q_e + (tid & ~63u);
     B* const qn = S.q_n + (tid & ~63u);
+    uint16_t* const qo = S.q_o + (tid & ~63u);
     uint32_t qc = 0;  // queued survivors of this wave (wave-uniform)
     // stage B for the queued survivors, one per lane
     auto drain = [&]() {
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");  // the wave's queue stores before its loads
         const bool live = lane < qc;
-        uint32_t ct = 0, ce = 0;
+        uint32_t ct = 0, ce = 0, cu = 0;
         B cnd = 0;
         if (live) {
             ct = qt[lane];
             ce = qe[lane];
             cnd = qn[lane];
+            cu = S.t_v[qo[lane]];
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");  // every lane has read before the queue is refilled
         const B old = update(live, ct, cnd);
-        apply(live, ce, ct, cnd, old);
+        apply(live, ce, ct, cnd, old, cu);
         qc = 0;
     };
     for (uint32_t base = 0; base < count; base += BS) {
@@ -534,6 +538,7 @@ __device__ __forceinline__ void expand_relax_compact(const uint32_t* list, uint3
                         qt[q] = st[j];
                         qe[q] = se[j];
                         qn[q] = snd[j];
+                        qo[q] = static_cast<uint16_t>(so[j]);
                     }
                 }
                 qc += nsv;
@@ -542,7 +547,7 @@ __device__ __forceinline__ void expand_relax_compact(const uint32_t* list, uint3
 #pragma unroll
                 for (int j = 0; j < UA; ++j) old[j] = update(sv[j], st[j], snd[j]);
 #pragma unroll
-                for (int j = 0; j < UA; ++j) apply(sv[j], se[j], st[j], snd[j], old[j]);
+                for (int j = 0; j < UA; ++j) apply(sv[j], se[j], st[j], snd[j], old[j], S.t_v[so[j]]);
             }
         }
         if (qc != 0u) drain();
@@ -577,6 +582,7 @@ struct KArgs {
     uint32_t* bits;
     uint64_t words;
     LogEntry* log;
+    uint32_t* log_hi;   // high halves of 8-byte candidate distances, parallel to log
     uint32_t logcap;
     uint32_t stride;    // nodes per slot in the workspace layout (>= n)
     // cohort launch (gtab != nullptr): graph table, item prefix per graph, item range per slot, flush layout
@@ -766,11 +772,13 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
     uint32_t* layers = nullptr;
     uint32_t* bits = nullptr;
     LogEntry* log = nullptr;
+    uint32_t* log_hi = nullptr;
     if constexpr (MODE != MODE_SSSP) {
         order = a.order + so;
         layers = a.layers + static_cast<size_t>(slot) * (stride + 1);
         bits = a.bits + static_cast<size_t>(slot) * a.words;
         log = a.log + static_cast<size_t>(slot) * a.logcap;
+        if constexpr (sizeof(B) == 8) log_hi = a.log_hi + static_cast<size_t>(slot) * a.logcap;
     }
     uint32_t* ctr = a.counters + static_cast<size_t>(slot) * 4;
     uint32_t phase = ctr[0], gen = ctr[1], pgen = ctr[2];
@@ -907,7 +915,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
         // distance before it); new frontier entries are stamped `nph`. In the betweenness modes every candidate
         // that could be tight (nd <= old) is logged: light arcs by the frontier phases (a node is expanded again
         // after each improvement, so also with its final distance), heavy arcs by the settle passes.
-        auto apply_core = [&](bool live, uint32_t e, uint32_t t, B nd, B old, uint32_t nph) {
+        auto apply_core = [&](bool live, uint32_t e, uint32_t t, B nd, B old, uint32_t nph, uint32_t u) {
             bool pf = false, pp = false, parked = false;
             if (nd < old) {
                 ++c_improve;
@@ -947,18 +955,19 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                     // loads there), so its lines leave the caches first and the distances and rows stay longer
                     if (lg && base + rank < a.logcap) {
                         const unsigned long long ndu = static_cast<unsigned long long>(nd);
-                        u32x4 q;  // LogEntry{t, e, nd} as one 16-byte store
+                        u32x4 q;  // LogEntry{t, e, u, low half of nd} as one 16-byte store
                         q.x = t;
                         q.y = e;
-                        q.z = static_cast<uint32_t>(ndu);
-                        q.w = static_cast<uint32_t>(ndu >> 32);
+                        q.z = u;
+                        q.w = static_cast<uint32_t>(ndu);
                         __builtin_nontemporal_store(q, reinterpret_cast<u32x4*>(log + base + rank));
+                        if constexpr (sizeof(B) == 8) __builtin_nontemporal_store(static_cast<uint32_t>(ndu >> 32), log_hi + base + rank);
                     }
                 }
             }
         };
-        auto apply = [&](bool ok, uint32_t, B dv, uint32_t e, uint32_t t, L l, bool final_t, B old, uint32_t nph) {
-            apply_core(ok && !final_t, e, t, T::add(dv, l), old, nph);
+        auto apply = [&](bool ok, uint32_t v, B dv, uint32_t e, uint32_t t, L l, bool final_t, B old, uint32_t nph) {
+            apply_core(ok && !final_t, e, t, T::add(dv, l), old, nph, v);
         };
         // the same steps for the compacted arc loop, given the candidate: the filter (heavy arc into a registered
         // node, or candidate above the target's bound) and the update
@@ -1001,7 +1010,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                 if constexpr (BS >= 512) {
                     expand_relax_compact<NG_PHASE_ARCS, BS>(
                         Fc, nF, S, tgt, len, light_row, [](bool, B) {}, [](B dv, L l) -> B { return T::add(dv, l); }, skip_arc, update_nd,
-                        [&](bool live, uint32_t e, uint32_t t, B nd, B old) { apply_core(live, e, t, nd, old, nph); });
+                        [&](bool live, uint32_t e, uint32_t t, B nd, B old, uint32_t u) { apply_core(live, e, t, nd, old, nph, u); });
                 } else {
                     expand_relax<NG_UNROLL, BS>(
                         Fc, nF, S, tgt, len, light_row, [](bool, B) {}, filter, update,
@@ -1116,7 +1125,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                     expand_relax_compact<kFilterArcs, BS>(
                         Sl + settle_from, nS - settle_from, S, tgt, len, settle_row, settle_tile,
                         [](B dv, L l) -> B { return T::add(dv, l); }, skip_arc, update_nd,
-                        [&](bool live, uint32_t e, uint32_t t, B nd, B old) { apply_core(live, e, t, nd, old, nph); });
+                        [&](bool live, uint32_t e, uint32_t t, B nd, B old, uint32_t u) { apply_core(live, e, t, nd, old, nph, u); });
                 } else {
                     expand_relax<NG_UNROLL, BS>(
                         Sl + settle_from, nS - settle_from, S, tgt, len, settle_row, settle_tile, filter, update,
@@ -1300,23 +1309,34 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
             // arc e into t is tight when its candidate c equals the final distance of t
             uint32_t n_tight = 0;
             auto tight = [&](uint32_t t, B c) -> bool { return !bd_skip(t, c) && c == ldc(DP(t)); };
+            // Once the distances are final the settled-node bitmap is free; it becomes the map of the nodes with
+            // a tight out-arc (`hc`). About two thirds to three quarters of a shortest-path tree's nodes are
+            // leaves, and the layers below skip a leaf's row bounds, bitmap words and dependency record.
+            uint32_t* const hc = bm;
+            if (use_bm) {
+                for (uint32_t w = tid; w < bmw; w += BS) hc[w] = 0u;
+                bsync();
+            }
+            auto internal = [&](uint32_t u) -> bool { return !use_bm || ((hc[u >> 5] >> (u & 31)) & 1u) != 0u; };
             // the frontier phases may log a light arc more than once: an arc counts when its bit is new
-            auto mark = [&](uint32_t t, uint32_t e, B c) {
+            auto mark = [&](uint32_t t, uint32_t e, B c, uint32_t u) {
                 if (tight(t, c)) {
                     const uint32_t bit = 1u << (e & 31);
-                    if ((wg_or(&bits[e >> 5], bit) & bit) == 0u) ++n_tight;
+                    if ((wg_or(&bits[e >> 5], bit) & bit) == 0u) {
+                        ++n_tight;
+                        if (use_bm) atomicOr(&hc[u >> 5], 1u << (u & 31));
+                    }
                 }
             };
             // every candidate (from the log, or from the full rows when the log overflowed) through fn
             auto each_candidate = [&](auto fn) {
                 if (nlog <= a.logcap) {
                     for (uint32_t i = tid; i < nlog; i += BS) {
-                        const u32x4 q = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(log + i));
-                        LogEntry r;
-                        r.t = q.x;
-                        r.e = q.y;
-                        r.nd = static_cast<unsigned long long>(q.z) | (static_cast<unsigned long long>(q.w) << 32);
-                        fn(r.t, r.e, static_cast<B>(r.nd));
+                        const u32x4 q = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(log + i));  // t, e, u, nd
+                        unsigned long long nd = q.w;
+                        if constexpr (sizeof(B) == 8)
+                            nd |= static_cast<unsigned long long>(__builtin_nontemporal_load(log_hi + i)) << 32;
+                        fn(q.x, q.y, static_cast<B>(nd), q.z);
                     }
                 } else {
                     expand<BS>(
@@ -1326,7 +1346,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                             beg = off[v];
                             cnt = off[v + 1] - beg;
                         },
-                        [&](uint32_t v, B, uint32_t e) { fn(tgt[e], e, T::add(ldc(DP(v)), len[e])); });
+                        [&](uint32_t v, B, uint32_t e) { fn(tgt[e], e, T::add(ldc(DP(v)), len[e]), v); });
                 }
             };
             if (nlog > a.logcap && tid == 0) atomicOr(a.err, ERR_LOG_OVERFLOW);
@@ -1351,6 +1371,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
             if (!tree) {  // tight in-degrees from the bitmap rows (each tight arc once)
                 for (uint32_t i = tid; i < nS; i += BS) {
                     const uint32_t u = Sl[i];
+                    if (!internal(u)) continue;
                     for_row_words(bits, off[u], off[u + 1], [&](uint32_t wi, uint32_t w) {
                         while (w) {
                             const uint32_t e = (wi << 5) + static_cast<uint32_t>(__ffs(w) - 1);
@@ -1376,6 +1397,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                 ++nl;
                 for (uint32_t i = lb + tid; i < le; i += BS) {
                     const uint32_t u = order[i];
+                    if (!internal(u)) continue;  // a leaf releases no children
                     if (tree) {
                         // a tree node releases each tight child: up to four children's targets are loaded
                         // together and appended with one LDS atomic
@@ -1460,7 +1482,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                     for (uint32_t i = l0 + tid; i < l1; i += BS) {
                         const uint32_t t = __builtin_nontemporal_load(order + i);
                         const uint32_t p = __builtin_nontemporal_load(par + i);
-                        const double d = ldc(&R[t].delta1);
+                        const double d = internal(t) ? ldc(&R[t].delta1) : 0.0;  // a leaf's dependency is 0
                         if (d != 0.0) {
                             R[t].bc += d;
                             if (rec_clean) stc(&R[t].delta1, 0.0);
@@ -1478,6 +1500,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                     const uint32_t l0 = layers[ly], l1 = layers[ly + 1];
                     for (uint32_t i = l0 + tid; i < l1; i += BS) {
                         const uint32_t u = order[i];
+                        // a leaf has no tight children: its dependencies stay 0 and its bitmap row is clear
+                        if (!internal(u)) continue;
                         const double su = tree ? 1.0 : ldc(&R[u].sigma);
                         double acc1 = 0.0, acc2 = 0.0;
                         // the tight children in increasing arc order, two at a time: both children's loads are
@@ -1487,8 +1511,10 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                             const uint32_t tb = two ? tgt[eb] : ta;
                             const double sta = tree ? 1.0 : ldc(&R[ta].sigma);
                             const double stb = tree ? 1.0 : ldc(&R[tb].sigma);
-                            const double da = ldc(&R[ta].delta1);
-                            const double db = ldc(&R[tb].delta1);
+                            // a leaf child's dependencies are 0 (on a tree its record is not read at all)
+                            const bool ia = internal(ta), ib = internal(tb);
+                            const double da = ia ? ldc(&R[ta].delta1) : 0.0;
+                            const double db = ib ? ldc(&R[tb].delta1) : 0.0;
                             uint32_t pa = 0, pb = 0;
                             if constexpr ((MODE & MODE_ARC1) != 0) {
                                 pa = perm[ea];
@@ -1498,8 +1524,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                             if constexpr ((MODE & MODE_ARC2) != 0) {
                                 dta = T::val(ldc(DP(ta)));
                                 dtb = T::val(ldc(DP(tb)));
-                                xa = ldc(&delta2[ta]);
-                                xb = ldc(&delta2[tb]);
+                                xa = ia ? ldc(&delta2[ta]) : 0.0;
+                                xb = ib ? ldc(&delta2[tb]) : 0.0;
                             }
                             if (rec_clean) {
                                 if (da != 0.0) stc(&R[ta].delta1, 0.0);
@@ -1755,6 +1781,7 @@ hipError_t launch_typed(const LaunchArgs& la, hipStream_t stream) {
     k.bits = ws.bits;
     k.words = ws.words;
     k.log = ws.log;
+    k.log_hi = ws.log_hi;
     k.logcap = ws.logcap;
     k.grad1 = ws.grad1;
     k.grad2 = ws.grad2;
