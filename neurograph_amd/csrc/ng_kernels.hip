@@ -1368,6 +1368,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
             // position (in the second frontier list, idle after the distances are final) and clear the bitmap.
             const bool push_dep = MODE == MODE_BC && tree;
             uint32_t* const par = Fb;
+            // ... and each internal node's number of children beside its own order position (in the first frontier
+            // list): a leaf's dependency is 0, so its term 1 reaches the parent as part of that count, not as a push
+            uint32_t* const nch = Fa;
             if (!tree) {  // tight in-degrees from the bitmap rows (each tight arc once)
                 for (uint32_t i = tid; i < nS; i += BS) {
                     const uint32_t u = Sl[i];
@@ -1401,8 +1404,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                     if (tree) {
                         // a tree node releases each tight child: up to four children's targets are loaded
                         // together and appended with one LDS atomic
-                        uint32_t e0 = 0, e1 = 0, e2 = 0, e3 = 0, k = 0;
+                        uint32_t e0 = 0, e1 = 0, e2 = 0, e3 = 0, k = 0, kids = 0;
                         auto flush = [&]() {
+                            kids += k;
                             const uint32_t t0 = tgt[e0];
                             const uint32_t t1 = k > 1 ? tgt[e1] : 0u;
                             const uint32_t t2 = k > 2 ? tgt[e2] : 0u;
@@ -1434,6 +1438,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                             }
                         });
                         if (k != 0u) flush();
+                        if (push_dep) __builtin_nontemporal_store(kids, nch + i);
                         continue;
                     }
                     const double su = ldc(&R[u].sigma);
@@ -1473,21 +1478,22 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
             NG_TICK(t_back);
             // ---- reverse layers: dependencies, per-node and per-arc accumulation, bitmap cleared
             if (push_dep) {
-                // A node's record is written only when its dependency is nonzero (the betweenness sum, and the
-                // dependency cleared for the next source, in one line): a leaf's record is read and left clean,
-                // and the reset below leaves every record alone.
+                // Only internal nodes take part: delta(t) = (number of children) + the dependencies its internal
+                // children pushed, all integers, so the atomic sums are exact in any order. A node's record is
+                // written only then (the betweenness sum, and the pushed part cleared for the next source, in one
+                // line): a leaf's record is neither read nor written, and the reset below leaves every record alone.
                 rec_clean = le == nS;
                 for (int ly = static_cast<int>(nl) - 1; ly >= 1; --ly) {
                     const uint32_t l0 = layers[ly], l1 = layers[ly + 1];
                     for (uint32_t i = l0 + tid; i < l1; i += BS) {
                         const uint32_t t = __builtin_nontemporal_load(order + i);
+                        if (!internal(t)) continue;
                         const uint32_t p = __builtin_nontemporal_load(par + i);
-                        const double d = internal(t) ? ldc(&R[t].delta1) : 0.0;  // a leaf's dependency is 0
-                        if (d != 0.0) {
-                            R[t].bc += d;
-                            if (rec_clean) stc(&R[t].delta1, 0.0);
-                        }
-                        if (ly > 1) wg_add(&R[p].delta1, 1.0 + d);
+                        const double pushed = ldc(&R[t].delta1);
+                        const double d = pushed + static_cast<double>(__builtin_nontemporal_load(nch + i));
+                        R[t].bc += d;
+                        if (rec_clean && pushed != 0.0) stc(&R[t].delta1, 0.0);
+                        if (ly > 1) wg_add(&R[p].delta1, d);
                     }
                     bsync();
                 }
