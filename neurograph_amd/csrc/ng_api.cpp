@@ -314,6 +314,15 @@ ng_status build_view_typed(ng_graph* g, int vi, DKind dk) {
         }
         NG_HIP(hipMemcpy(v.len, slen.data(), sizeof(L) * m, hipMemcpyHostToDevice));
     }
+    if constexpr (sizeof(L) == 4) {
+        if (v.arcs == nullptr) NG_HIP(hipMalloc(&v.arcs, 8 * std::max<uint64_t>(m, 1)));
+        if (m) {
+            NG_HIP(launch_interleave_arcs(v.tgt, static_cast<const uint32_t*>(v.len), m, v.arcs, g->ctx->stream));
+            NG_HIP(hipStreamSynchronize(g->ctx->stream));
+        }
+    } else {
+        dfree(v.arcs);  // 8-byte lengths are read from tgt and len
+    }
     v.dk = dk;
     v.min_len = fin ? mn : 0.0;
     v.max_len = mx;
@@ -365,6 +374,7 @@ void free_view(View& v) {
     v.len = nullptr;
     dfree(v.perm);
     dfree(v.rows);
+    dfree(v.arcs);
     std::vector<uint32_t>().swap(v.perm_host);
     v.built = false;
 }
@@ -1464,7 +1474,7 @@ ng_status cohort_build(ng_cohort* c) {
         double d = pick_delta(g, v, 0, 1, c->delta[gi]);
         if (c->dk == DKind::U32 || c->dk == DKind::U64) d = std::max(1.0, std::round(d));
         eff_delta[gi] = d;
-        gtab[gi] = CohortGraph{v.off, v.tgt, v.len, nullptr, v.perm, g->n, 0u, d};
+        gtab[gi] = CohortGraph{v.off, v.tgt, v.len, v.arcs, nullptr, v.perm, g->n, 0u, d};
     }
     c->total = c->prefix[G];
     c->versions.resize(G);

@@ -64,6 +64,8 @@ struct View {
     uint32_t* off = nullptr;   // [n+1]
     uint32_t* tgt = nullptr;   // [m]
     void* len = nullptr;       // [m], element type dk
+    uint32_t* arcs = nullptr;  // [2m] (target, length bits) of every sorted arc, for 4-byte lengths: one 8-byte load
+                               // per arc and one range of lines per row instead of two
     uint32_t* perm = nullptr;  // [m]
     double min_len = 0, max_len = 0, mean_len = 0, q05_len = 0;  // over finite lengths
     bool has_zero = false;
@@ -152,6 +154,7 @@ struct CohortGraph {
     const uint32_t* off;
     const uint32_t* tgt;
     const void* len;
+    const uint32_t* arcs;
     const RowRange* rows;
     const uint32_t* perm;
     uint32_t n;
@@ -186,6 +189,8 @@ struct LaunchArgs {
 hipError_t launch_batch(const LaunchArgs& a, hipStream_t stream);
 hipError_t launch_light_end(DKind dk, uint32_t n, const uint32_t* off, const void* len, double delta, RowRange* rows,
                             hipStream_t stream);
+// arcs[2k] = tgt[k], arcs[2k+1] = len[k] (4-byte lengths as bits), k < m
+hipError_t launch_interleave_arcs(const uint32_t* tgt, const uint32_t* len, uint64_t m, uint32_t* arcs, hipStream_t stream);
 hipError_t launch_reduce_slots(const BcRec* rec, uint32_t slots, uint32_t n, uint32_t stride, double* out, hipStream_t stream);
 // Sets every record to the clean state (all zero).
 hipError_t launch_rec_init(BcRec* rec, uint64_t count, hipStream_t stream);

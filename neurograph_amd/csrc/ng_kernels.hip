@@ -142,6 +142,28 @@ __device__ __forceinline__ gptr<T> as_global(const T* p) {
 }
 // A node's row record (RowRange: first arc, light-prefix end, end) as one 16-byte load.
 using u32x4 = uint32_t __attribute__((ext_vector_type(4)));
+using u32x2 = uint32_t __attribute__((ext_vector_type(2)));
+// Target and length of sorted arc e: one 8-byte load of the interleaved pair for 4-byte lengths, else two loads.
+#ifndef NG_PAIRS_MAX_BS
+#define NG_PAIRS_MAX_BS 256
+#endif
+// Whether a kernel with BS lanes per workgroup and lengths of type L reads the interleaved pairs.
+template <class L, int BS>
+constexpr bool kPairs = sizeof(L) == 4 && BS <= NG_PAIRS_MAX_BS;
+template <class L, int BS>
+__device__ __forceinline__ void load_arc(gptr<uint32_t> tgt, gptr<L> len, gptr<u32x2> arcs, uint32_t e, uint32_t& t, L& l) {
+    if constexpr (kPairs<L, BS>) {
+        const u32x2 p = arcs[e];
+        t = p.x;
+        // the element is copied to a scalar first: hipcc's __builtin_bit_cast of a vector element (p.y) reads
+        // element 0 (checked in the generated code)
+        const uint32_t ly = p.y;
+        l = __builtin_bit_cast(L, ly);
+    } else {
+        t = tgt[e];
+        l = len[e];
+    }
+}
 __device__ __forceinline__ u32x4 load_row(gptr<RowRange> rows, uint32_t v) { return ((gptr<u32x4>)rows)[v]; }
 // Phase boundary: a workgroup barrier with workgroup-scope release/acquire.
 __device__ __forceinline__ void bsync() { __syncthreads(); }
@@ -299,8 +321,8 @@ __device__ __forceinline__ void expand(const uint32_t* list, uint32_t count, Lds
 #endif
 template <int U, int BS, class B, class L, class NodeFn, class TileFn, class FilterFn, class UpdateFn, class ApplyFn>
 __device__ __forceinline__ void expand_relax(const uint32_t* list, uint32_t count, Lds<B, BS>& S, gptr<uint32_t> tgt,
-                                             gptr<L> len, NodeFn node_fn, TileFn tile_fn, FilterFn filter, UpdateFn update,
-                                             ApplyFn apply) {
+                                             gptr<L> len, gptr<u32x2> arcs, NodeFn node_fn, TileFn tile_fn, FilterFn filter,
+                                             UpdateFn update, ApplyFn apply) {
     const uint32_t tid = threadIdx.x;
     for (uint32_t base = 0; base < count; base += BS) {
         const uint32_t i = base + tid;
@@ -358,10 +380,7 @@ __device__ __forceinline__ void expand_relax(const uint32_t* list, uint32_t coun
             }
 #pragma unroll
             for (int j = 0; j < U; ++j) {
-                if (pok[j]) {
-                    pt[j] = tgt[pe[j]];
-                    pl[j] = len[pe[j]];
-                }
+                if (pok[j]) load_arc<L, BS>(tgt, len, arcs, pe[j], pt[j], pl[j]);
             }
 #pragma unroll
             for (int j = 0; j < U; ++j) {
@@ -425,8 +444,8 @@ __device__ __forceinline__ uint32_t code4(double x) { return x < 15.0 ? static_c
 // also receives the arc's tail u.
 template <int UA, int BS, class B, class L, class NodeFn, class TileFn, class AddFn, class SkipFn, class UpdateFn, class ApplyFn>
 __device__ __forceinline__ void expand_relax_compact(const uint32_t* list, uint32_t count, Lds<B, BS>& S, gptr<uint32_t> tgt,
-                                                     gptr<L> len, NodeFn node_fn, TileFn tile_fn, AddFn add, SkipFn skip,
-                                                     UpdateFn update, ApplyFn apply) {
+                                                     gptr<L> len, gptr<u32x2> arcs, NodeFn node_fn, TileFn tile_fn, AddFn add,
+                                                     SkipFn skip, UpdateFn update, ApplyFn apply) {
     static_assert(BS >= 512, "the survivor queues are laid out for wide workgroups");
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = lane_id();
@@ -505,10 +524,7 @@ __device__ __forceinline__ void expand_relax_compact(const uint32_t* list, uint3
             for (int j = 0; j < UA; ++j) {
                 st[j] = 0;
                 sl[j] = L(0);
-                if (sv[j]) {
-                    st[j] = tgt[se[j]];
-                    sl[j] = len[se[j]];
-                }
+                if (sv[j]) load_arc<L, BS>(tgt, len, arcs, se[j], st[j], sl[j]);
             }
 #pragma unroll
             for (int j = 0; j < UA; ++j) {
@@ -563,6 +579,7 @@ struct KArgs {
     const uint32_t* off;
     const uint32_t* tgt;
     const L* len;
+    const uint32_t* arcs;  // interleaved (target, length) pairs, 4-byte lengths only
     const RowRange* rows;
     const uint32_t* perm;
     const uint32_t* sources;
@@ -752,6 +769,14 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
     gptr<uint32_t> off = as_global(a.off);
     gptr<uint32_t> tgt = as_global(a.tgt);
     gptr<L> len = as_global(a.len);
+    gptr<u32x2> arcs = (gptr<u32x2>)as_global(a.arcs);
+    // the target of sorted arc e, from the interleaved pairs where they exist: the passes then read one copy of the
+    // graph's arcs, which the last-level cache can hold (spatial200k: 160 MB of pairs; pairs plus the separate
+    // targets would be 240 MB of the 256 MB MALL)
+    auto target_of = [&](uint32_t e) -> uint32_t {
+        if constexpr (kPairs<L, BS>) return arcs[e].x;
+        else return tgt[e];
+    };
     gptr<RowRange> rows = as_global(a.rows);
     gptr<uint32_t> perm = as_global(a.perm);
     uint32_t bmw = a.bm_words;  // bitmap words of the current graph
@@ -821,6 +846,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                 off = as_global(cg.off);
                 tgt = as_global(cg.tgt);
                 len = as_global(static_cast<const L*>(cg.len));
+                arcs = (gptr<u32x2>)as_global(cg.arcs);
                 rows = as_global(cg.rows);
                 perm = as_global(cg.perm);
                 delta = static_cast<L>(cg.delta);
@@ -1009,11 +1035,11 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                 };
                 if constexpr (BS >= 512) {
                     expand_relax_compact<NG_PHASE_ARCS, BS>(
-                        Fc, nF, S, tgt, len, light_row, [](bool, B) {}, [](B dv, L l) -> B { return T::add(dv, l); }, skip_arc, update_nd,
+                        Fc, nF, S, tgt, len, arcs, light_row, [](bool, B) {}, [](B dv, L l) -> B { return T::add(dv, l); }, skip_arc, update_nd,
                         [&](bool live, uint32_t e, uint32_t t, B nd, B old, uint32_t u) { apply_core(live, e, t, nd, old, nph, u); });
                 } else {
                     expand_relax<NG_UNROLL, BS>(
-                        Fc, nF, S, tgt, len, light_row, [](bool, B) {}, filter, update,
+                        Fc, nF, S, tgt, len, arcs, light_row, [](bool, B) {}, filter, update,
                         [&](bool ok, uint32_t v, B dv, uint32_t e, uint32_t t, L l, bool fin, B old) {
                             apply(ok, v, dv, e, t, l, fin, old, nph);
                         });
@@ -1123,12 +1149,12 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                     // +1 % on spatial50k), more for 4-byte ones (6 vs 4: +0.6 % on spatial200k)
                     constexpr int kFilterArcs = sizeof(B) == 8 ? NG_SETTLE_ARCS_8B : NG_SETTLE_ARCS_4B;
                     expand_relax_compact<kFilterArcs, BS>(
-                        Sl + settle_from, nS - settle_from, S, tgt, len, settle_row, settle_tile,
+                        Sl + settle_from, nS - settle_from, S, tgt, len, arcs, settle_row, settle_tile,
                         [](B dv, L l) -> B { return T::add(dv, l); }, skip_arc, update_nd,
                         [&](bool live, uint32_t e, uint32_t t, B nd, B old, uint32_t u) { apply_core(live, e, t, nd, old, nph, u); });
                 } else {
                     expand_relax<NG_UNROLL, BS>(
-                        Sl + settle_from, nS - settle_from, S, tgt, len, settle_row, settle_tile, filter, update,
+                        Sl + settle_from, nS - settle_from, S, tgt, len, arcs, settle_row, settle_tile, filter, update,
                         [&](bool ok, uint32_t v, B dv, uint32_t e, uint32_t t, L l, bool fin, B old) {
                             apply(ok, v, dv, e, t, l, fin, old, nph);
                         });
@@ -1346,7 +1372,12 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                             beg = off[v];
                             cnt = off[v + 1] - beg;
                         },
-                        [&](uint32_t v, B, uint32_t e) { fn(tgt[e], e, T::add(ldc(DP(v)), len[e]), v); });
+                        [&](uint32_t v, B, uint32_t e) {
+                            uint32_t t;
+                            L l;
+                            load_arc<L, BS>(tgt, len, arcs, e, t, l);
+                            fn(t, e, T::add(ldc(DP(v)), l), v);
+                        });
                 }
             };
             if (nlog > a.logcap && tid == 0) atomicOr(a.err, ERR_LOG_OVERFLOW);
@@ -1379,7 +1410,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                         while (w) {
                             const uint32_t e = (wi << 5) + static_cast<uint32_t>(__ffs(w) - 1);
                             w &= w - 1;
-                            wg_add(&R[tgt[e]].indeg, 1u);
+                            wg_add(&R[target_of(e)].indeg, 1u);
                         }
                     });
                 }
@@ -1407,10 +1438,10 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                         uint32_t e0 = 0, e1 = 0, e2 = 0, e3 = 0, k = 0, kids = 0;
                         auto flush = [&]() {
                             kids += k;
-                            const uint32_t t0 = tgt[e0];
-                            const uint32_t t1 = k > 1 ? tgt[e1] : 0u;
-                            const uint32_t t2 = k > 2 ? tgt[e2] : 0u;
-                            const uint32_t t3 = k > 3 ? tgt[e3] : 0u;
+                            const uint32_t t0 = target_of(e0);
+                            const uint32_t t1 = k > 1 ? target_of(e1) : 0u;
+                            const uint32_t t2 = k > 2 ? target_of(e2) : 0u;
+                            const uint32_t t3 = k > 3 ? target_of(e3) : 0u;
                             const uint32_t pos = atomicAdd(&S.cnt[4], k);
                             auto put = [&](uint32_t j, uint32_t t) {
                                 if (pos + j < n) {
@@ -1446,7 +1477,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                         while (w) {
                             const uint32_t e = (wi << 5) + static_cast<uint32_t>(__ffs(w) - 1);
                             w &= w - 1;
-                            const uint32_t t = tgt[e];
+                            const uint32_t t = target_of(e);
                             wg_add(&R[t].sigma, su);
                             const bool released = wg_sub(&R[t].indeg, 1u) == 1u;
                             if (released) {
@@ -1513,8 +1544,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                         // the tight children in increasing arc order, two at a time: both children's loads are
                         // in flight together, and the sums still add the terms in arc order
                         auto visit = [&](uint32_t ea, uint32_t eb, bool two) {
-                            const uint32_t ta = tgt[ea];
-                            const uint32_t tb = two ? tgt[eb] : ta;
+                            const uint32_t ta = target_of(ea);
+                            const uint32_t tb = two ? target_of(eb) : ta;
                             const double sta = tree ? 1.0 : ldc(&R[ta].sigma);
                             const double stb = tree ? 1.0 : ldc(&R[tb].sigma);
                             // a leaf child's dependencies are 0 (on a tree its record is not read at all)
@@ -1646,6 +1677,16 @@ __global__ void light_end_kernel(uint32_t n, const uint32_t* off, const L* len, 
     rows[v] = RowRange{beg, lo, end, 0u};
 }
 
+__global__ void interleave_arcs_kernel(const uint32_t* tgt, const uint32_t* len, uint64_t m, u32x2* arcs) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < m; i += stride) {
+        u32x2 p;
+        p.x = tgt[i];
+        p.y = len[i];
+        arcs[i] = p;
+    }
+}
+
 // out[v] = sum over slots of the betweenness partial of node v, in slot order (deterministic)
 __global__ void reduce_slots_kernel(const BcRec* rec, uint32_t slots, uint32_t n, uint32_t stride, double* out) {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1756,6 +1797,7 @@ hipError_t launch_typed(const LaunchArgs& la, hipStream_t stream) {
         k.off = vw.off;
         k.tgt = vw.tgt;
         k.len = static_cast<const L*>(vw.len);
+        k.arcs = vw.arcs;
         k.rows = vw.rows;
         k.perm = vw.perm;
     } else {
@@ -2010,6 +2052,14 @@ hipError_t launch_cohort_reduce(const double* flush, const uint32_t* node_prefix
     if (total_nodes == 0) return hipSuccess;
     hipLaunchKernelGGL(cohort_reduce_kernel, dim3((total_nodes + 255) / 256), dim3(256), 0, stream, flush, node_prefix, n_graphs,
                        seg_ptr, seg_off, total_nodes, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_interleave_arcs(const uint32_t* tgt, const uint32_t* len, uint64_t m, uint32_t* arcs, hipStream_t stream) {
+    if (m == 0) return hipSuccess;
+    const uint64_t blocks64 = (m + 255) / 256;
+    const uint32_t blocks = static_cast<uint32_t>(blocks64 < 65536 ? blocks64 : 65536);
+    hipLaunchKernelGGL(interleave_arcs_kernel, dim3(blocks), dim3(256), 0, stream, tgt, len, m, reinterpret_cast<u32x2*>(arcs));
     return hipGetLastError();
 }
 
