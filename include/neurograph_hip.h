@@ -81,6 +81,8 @@ typedef struct {
                                   built with -DNG_SECTION_TIMERS */
     uint64_t tree_sources;   /* betweenness modes: sources whose tight arcs formed a shortest-path tree (every sigma
                                 1: the tree layers ran) */
+    uint64_t near_tree_sources; /* Brandes mode: sources whose tight arcs formed a tree but for a few nodes with
+                                   several tight in-arcs (the near-tree layers ran) */
 } ng_call_stats;
 
 typedef struct ng_context ng_context;

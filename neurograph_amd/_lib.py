@@ -50,7 +50,8 @@ class CallStats(ctypes.Structure):
                 ("block", ctypes.c_uint32), ("phases", ctypes.c_uint64), ("buckets", ctypes.c_uint64),
                 ("light_arcs", ctypes.c_uint64), ("settle_arcs", ctypes.c_uint64), ("logged_arcs", ctypes.c_uint64),
                 ("layers", ctypes.c_uint64), ("probes", ctypes.c_uint64), ("improvements", ctypes.c_uint64),
-                ("section_ticks", ctypes.c_uint64 * 8), ("tree_sources", ctypes.c_uint64)]
+                ("section_ticks", ctypes.c_uint64 * 8), ("tree_sources", ctypes.c_uint64),
+                ("near_tree_sources", ctypes.c_uint64)]
 
 
 class HostCsrC(ctypes.Structure):

@@ -680,6 +680,7 @@ ng_status run_batch(ng_graph* g, const RunSpec& rs, uint32_t* slots_used, double
     NG_HIP(hipMemcpy(ctrs, w.arcs_reached, 8 * 32, hipMemcpyDeviceToHost));
     for (int sct = 0; sct < 8; ++sct) g->last.section_ticks[sct] = ctrs[9 + sct];
     g->last.tree_sources = ctrs[17];
+    g->last.near_tree_sources = ctrs[18];
     const unsigned long long arcs = ctrs[0];
     if (rs.dist_out) {
         NG_HIP(hipMemcpy(rs.dist_out, d_dist_out, dob * static_cast<size_t>(rs.n_sources) * g->n, hipMemcpyDeviceToHost));
@@ -1267,6 +1268,7 @@ ng_status ng_brandes_accumulate_noalloc(ng_graph* g, const uint32_t* sources, ui
     l.improvements = ctrs[8];
     for (int sct = 0; sct < 8; ++sct) l.section_ticks[sct] = ctrs[9 + sct];
     l.tree_sources = ctrs[17];
+    l.near_tree_sources = ctrs[18];
     return NG_OK;
 }
 
@@ -1631,6 +1633,7 @@ ng_status cohort_replay(ng_cohort* c) {
     l.improvements = ctrs[8];
     for (int sct = 0; sct < 8; ++sct) l.section_ticks[sct] = ctrs[9 + sct];
     l.tree_sources = ctrs[17];
+    l.near_tree_sources = ctrs[18];
     return NG_OK;
 }
 

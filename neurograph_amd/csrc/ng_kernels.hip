@@ -786,6 +786,18 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
     uint32_t* sstamp = a.sstamp + so;
     BcRec* const R = (MODE != MODE_SSSP) ? a.rec + so : nullptr;
     double* const delta2 = ((MODE & MODE_ARC2) != 0) ? a.delta2 + so : nullptr;
+    // Near trees: Brandes sources whose tight arcs form a tree except at a few nodes with several tight in-arcs
+    // (f32 rounding makes two path lengths equal now and then: 13 % of the 200k-node graph's sources). They keep
+    // the tree layers' costs: verification flags the first tight in-arc of every node in the top bit of its
+    // distance (never set in a reached node's distance of these types) and lists the extra ones; only those nodes
+    // take sigma sums and in-degrees. Wide workgroups (their survivor queues hold the lists), Brandes mode, f32
+    // distances (f64 ties are rare, and the kernels of the other types measured slower with the extra code); sigma
+    // by order position goes to the second-dependency array, idle in this mode.
+    constexpr bool kNear = MODE == MODE_BC && BS >= 512 && !LD && std::is_same_v<D, float>;
+    constexpr B kTop = static_cast<B>(B(1) << (8 * sizeof(B) - 1));
+    const bool near_ok = kNear && use_bd && a.delta2 != nullptr;
+    double* const sigp = near_ok ? a.delta2 + so : nullptr;
+    const B top = near_ok ? kTop : B(0);
     uint32_t* lists = a.lists + so * 3;
     uint32_t* const Fa = lists;
     uint32_t* const Fb = lists + stride;
@@ -811,6 +823,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
     // work counters (uniform per workgroup, except the arc counts which lane 0 accumulates from the tiles)
     unsigned long long c_phases = 0, c_buckets = 0, c_logged = 0, c_layers = 0;
     unsigned long long c_trees = 0;  // sources whose tight arcs formed a shortest-path tree
+    unsigned long long c_near = 0;   // sources whose tight arcs formed a near tree
     // per-lane counts of one launch (32 bits: a lane handles far fewer than 2^32 arcs per launch)
     uint32_t c_light = 0, c_settle = 0, c_probe = 0, c_improve = 0;
 
@@ -1334,7 +1347,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
             c_logged += nlog;
             // arc e into t is tight when its candidate c equals the final distance of t
             uint32_t n_tight = 0;
-            auto tight = [&](uint32_t t, B c) -> bool { return !bd_skip(t, c) && c == ldc(DP(t)); };
+            auto tight = [&](uint32_t t, B c) -> bool { return !bd_skip(t, c) && c == (ldc(DP(t)) & ~top); };
             // Once the distances are final the settled-node bitmap is free; it becomes the map of the nodes with
             // a tight out-arc (`hc`). About two thirds to three quarters of a shortest-path tree's nodes are
             // leaves, and the layers below skip a leaf's row bounds, bitmap words and dependency record.
@@ -1351,6 +1364,13 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                     if ((wg_or(&bits[e >> 5], bit) & bit) == 0u) {
                         ++n_tight;
                         if (use_bm) atomicOr(&hc[u >> 5], 1u << (u & 31));
+                        if constexpr (kNear) {
+                            // the first tight in-arc of t sets the flag; each later one lists t once more
+                            if (near_ok && (wg_or(DP(t), top) & top) != B(0)) {
+                                const uint32_t k = atomicAdd(&S.cnt[7], 1u);
+                                if (k < Lds<B, BS>::kQ) S.q_t[k] = t;
+                            }
+                        }
                     }
                 }
             };
@@ -1376,7 +1396,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                             uint32_t t;
                             L l;
                             load_arc<L, BS>(tgt, len, arcs, e, t, l);
-                            fn(t, e, T::add(ldc(DP(v)), l), v);
+                            fn(t, e, T::add(ldc(DP(v)) & ~top, l), v);
                         });
                 }
             };
@@ -1402,7 +1422,27 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
             // ... and each internal node's number of children beside its own order position (in the first frontier
             // list): a leaf's dependency is 0, so its term 1 reaches the parent as part of that count, not as a push
             uint32_t* const nch = Fa;
-            if (!tree) {  // tight in-degrees from the bitmap rows (each tight arc once)
+            // A near tree: its extra tight arcs (one per listed node entry) fit the list, and their nodes with the
+            // arcs into them fit the second list (at most twice as many entries). The bounds' LDS becomes the map of
+            // the nodes with several tight in-arcs (`multi`), whose in-degrees go to their records.
+            const uint32_t n_extra = S.cnt[7];
+            const bool near = kNear && near_ok && !tree && 2u * n_extra <= Lds<B, BS>::kQ;
+            uint32_t* const multi = reinterpret_cast<uint32_t*>(fl);
+            auto is_multi = [&](uint32_t v) -> bool { return ((multi[v >> 5] >> (v & 31)) & 1u) != 0u; };
+            if constexpr (kNear) if (near) {
+                for (uint32_t w = tid; w < bmw; w += BS) multi[w] = 0u;
+                if (tid == 0) S.cnt[1] = 0;  // entries of the list of arcs into multi nodes (Kahn)
+                bsync();
+                for (uint32_t k = tid; k < n_extra; k += BS) {
+                    const uint32_t t = S.q_t[k];
+                    const uint32_t bit = 1u << (t & 31);
+                    // in-degree = listed entries + the first tight in-arc
+                    wg_add(&R[t].indeg, (atomicOr(&multi[t >> 5], bit) & bit) == 0u ? 2u : 1u);
+                }
+                bsync();
+            }
+            c_near += near ? 1u : 0u;
+            if (!tree && !near) {  // tight in-degrees from the bitmap rows (each tight arc once)
                 for (uint32_t i = tid; i < nS; i += BS) {
                     const uint32_t u = Sl[i];
                     if (!internal(u)) continue;
@@ -1472,6 +1512,45 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                         if (push_dep) __builtin_nontemporal_store(kids, nch + i);
                         continue;
                     }
+                    if constexpr (kNear) if (near) {
+                        // A near-tree node: sigma by order position (the source's is 1; a multi node's, marked -1, is
+                        // the sum in its record). A child with one tight in-arc is released at once with the
+                        // parent's sigma; a multi child adds the parent's sigma to its record, lists the arc and is
+                        // released by its last parent. The bitmap row is cleared here. nch: single-parent leaves.
+                        double su = i == 0 ? 1.0 : sigp[i];
+                        if (su < 0.0) su = ldc(&R[u].sigma);
+                        uint32_t nsl = 0;
+                        auto release = [&](uint32_t t, double st) {
+                            const uint32_t pos = atomicAdd(&S.cnt[4], 1u);
+                            if (pos < n) {
+                                order[pos] = t;
+                                par[pos] = u;
+                                sigp[pos] = st;
+                            }
+                        };
+                        for_row_words(bits, off[u], off[u + 1], [&](uint32_t wi, uint32_t w) {
+                            wg_and(&bits[wi], ~w);
+                            while (w) {
+                                const uint32_t e = (wi << 5) + static_cast<uint32_t>(__ffs(w) - 1);
+                                w &= w - 1;
+                                const uint32_t t = target_of(e);
+                                if (is_multi(t)) {
+                                    wg_add(&R[t].sigma, su);
+                                    const uint32_t k = atomicAdd(&S.cnt[1], 1u);
+                                    if (k < Lds<B, BS>::kQ) {
+                                        S.q_e[k] = t;
+                                        S.q_n[k] = static_cast<B>(u);
+                                    }
+                                    if (wg_sub(&R[t].indeg, 1u) == 1u) release(t, -1.0);
+                                } else {
+                                    nsl += internal(t) ? 0u : 1u;
+                                    release(t, su);
+                                }
+                            }
+                        });
+                        nch[i] = nsl;
+                        continue;
+                    }
                     const double su = ldc(&R[u].sigma);
                     for_row_words(bits, off[u], off[u + 1], [&](uint32_t wi, uint32_t w) {
                         while (w) {
@@ -1525,6 +1604,44 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                         R[t].bc += d;
                         if (rec_clean && pushed != 0.0) stc(&R[t].delta1, 0.0);
                         if (ly > 1) wg_add(&R[p].delta1, d);
+                    }
+                    bsync();
+                }
+            } else if (kNear && near) {
+                // Near tree, pushed the other way: a node t other than a single-parent leaf takes
+                // delta(t) = sigma(t) (pushed + nch / sigma(t)), where `pushed` holds the terms
+                // y(c) = (1 + delta(c)) / sigma(c) of its internal single-parent children and its multi children, and
+                // each single-parent leaf child contributes 1 / sigma(t); it then sends y(t) to its parent, or to each
+                // of its parents from the list when it is a multi node: the per-arc term sigma(p) (1 + delta(t)) /
+                // sigma(t) of BrandesVariant.h:151-154, with sigma(p) applied at p. Records are cleaned as on a tree.
+                rec_clean = le == nS;
+                const uint32_t n_arcs = S.cnt[1];
+                for (int ly = static_cast<int>(nl) - 1; ly >= 1; --ly) {
+                    const uint32_t l0 = layers[ly], l1 = layers[ly + 1];
+                    for (uint32_t i = l0 + tid; i < l1; i += BS) {
+                        const uint32_t t = order[i];
+                        const bool mt = is_multi(t), it = internal(t);
+                        if (!it && !mt) continue;
+                        double st = sigp[i];
+                        if (st < 0.0) st = ldc(&R[t].sigma);
+                        double dlt = 0.0;
+                        if (it) {
+                            const double pushed = ldc(&R[t].delta1);
+                            dlt = st * (pushed + static_cast<double>(nch[i]) / st);
+                            R[t].bc += dlt;
+                            if (rec_clean && pushed != 0.0) stc(&R[t].delta1, 0.0);
+                        }
+                        const double y = (1.0 + dlt) / st;
+                        if (mt) {
+                            for (uint32_t k = 0; k < n_arcs; ++k) {
+                                if (S.q_e[k] != t) continue;
+                                const uint32_t p = static_cast<uint32_t>(S.q_n[k]);
+                                if (p != s) wg_add(&R[p].delta1, y);
+                            }
+                            if (rec_clean) stc(&R[t].sigma, 0.0);
+                        } else if (ly > 1) {
+                            wg_add(&R[par[i]].delta1, y);
+                        }
                     }
                     bsync();
                 }
@@ -1615,7 +1732,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
         NG_TICK(t_reset);
         if (a.dist_out != nullptr) {
             for (uint32_t v = tid; v < n; v += BS)
-                write_dist_out(a, static_cast<size_t>(si) * n + v, ldc(DP(v)));
+                write_dist_out(a, static_cast<size_t>(si) * n + v, ldc(DP(v)) & ~top);
             bsync();  // every lane has read its distances before any lane resets them below
         }
         // ---- restore the slot: only reached nodes were touched (after an abort, frontier and pile nodes
@@ -1648,6 +1765,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
         atomicAdd(&a.arcs_reached[5], c_logged);
         atomicAdd(&a.arcs_reached[6], c_layers);
         atomicAdd(&a.arcs_reached[17], c_trees);
+        atomicAdd(&a.arcs_reached[18], c_near);
     }
     // per-lane arc counts (each lane counted the rows it loaded)
     auto add64 = [](unsigned long long x, unsigned long long y) { return x + y; };

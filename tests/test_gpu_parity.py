@@ -459,3 +459,62 @@ def test_graph_options_are_validated(ctx):
             g.set_options(**{name: bad})
         assert e.value.status == ng.NG_EINVAL
     assert g.get_option("lds_maps") == -1 and g.get_option("block") == 0
+
+
+def gadget_graph(n, m, n_gadgets, seed, wdtype):
+    """A random graph of lognormal lengths (all >= 0.2) plus `n_gadgets` tie gadgets: from a random node x, arcs of
+    length 1/8 into fresh nodes a and b (and c for every third gadget) and from each of them into a fresh node t, so
+    that every source reaching x finds t at the same distance over two or three paths (sums of exact binary
+    fractions): t has several tight in-arcs, while the rest of the shortest-path DAG stays a tree. Every t gets
+    out-arcs into the random part, and every fifth gadget hangs below the previous one's t, so that multi nodes are
+    internal, leaves, and nested (sigma 4 and more)."""
+    rng = np.random.default_rng(seed)
+    src = list(rng.integers(0, n, size=m))
+    dst = list(rng.integers(0, n, size=m))
+    w = list(0.2 + np.exp(rng.normal(0.0, 0.5, size=m)))
+    total = n
+    prev_t = None
+    for k in range(n_gadgets):
+        x = prev_t if (k % 5 == 4 and prev_t is not None) else int(rng.integers(0, n))
+        mids = [total + j for j in range(3 if k % 3 == 2 else 2)]
+        t = total + len(mids)
+        total = t + 1
+        for a in mids:
+            src += [x, a]
+            dst += [a, t]
+            w += [0.125, 0.125]
+        if k % 2 == 0:  # internal multi node
+            for y in rng.integers(0, n, size=3):
+                src.append(t)
+                dst.append(int(y))
+                w.append(0.2 + float(np.exp(rng.normal(0.0, 0.5))))
+        prev_t = t
+    return orc.Csr.from_edges(total, np.array(src), np.array(dst), np.array(w, dtype=wdtype), dtype=wdtype)
+
+
+@pytest.mark.parametrize("wdtype", [np.float32, np.float64])
+def test_near_tree_sources(ctx, wdtype):
+    """Sources whose tight arcs form a tree except at a few nodes with several tight in-arcs (tie gadgets) take the
+    near-tree layers in 1024-lane workgroups with LDS bounds and f32 distances; with more such nodes than the lists
+    hold, or f64 distances, they take the general layers. All against the oracle: betweenness within 1e-9, distances
+    bit-exact, and the slots clean for the next calls (a second Brandes call and SSSP give the same values)."""
+    for n_gadgets, expect_near in ((7, wdtype == np.float32), (700, False)):
+        c = gadget_graph(3000, 24000, n_gadgets, 90 + n_gadgets, wdtype)
+        srcs = np.arange(0, 3000, 11, dtype=np.uint32)
+        ref = orc.brandes(c, sources=srcs, want_dist=True)
+        for maps in (4, 8):  # 16-bit bounds, 4-bit codes
+            g = dev_graph(ctx, c, block=1024, lds_maps=maps)
+            bc = g.brandes(srcs)
+            st = g.last_stats()
+            assert st["block"] == 1024
+            if expect_near:
+                assert st["near_tree_sources"] > 0.5 * len(srcs), st
+            else:
+                assert st["near_tree_sources"] == 0, st
+            np.testing.assert_allclose(bc, ref["bc"], rtol=1e-9, atol=1e-9)
+            np.testing.assert_allclose(g.brandes(srcs), bc, rtol=1e-12, atol=1e-12)
+            _, dist = g.sssp(srcs, want_dist=True)
+            np.testing.assert_array_equal(dist, ref["dist"])
+            # edge path counts (general layers, no near-tree path) after the near-tree calls
+            np.testing.assert_allclose(g.edge_dependency(srcs), orc.brandes(c, sources=srcs, edge=True)["edge"],
+                                       rtol=1e-9, atol=1e-12)
