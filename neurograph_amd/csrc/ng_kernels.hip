@@ -788,16 +788,14 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
     double* const delta2 = ((MODE & MODE_ARC2) != 0) ? a.delta2 + so : nullptr;
     // Near trees: Brandes sources whose tight arcs form a tree except at a few nodes with several tight in-arcs
     // (f32 rounding makes two path lengths equal now and then: 13 % of the 200k-node graph's sources). They keep
-    // the tree layers' costs: verification flags the first tight in-arc of every node in the top bit of its
-    // distance (never set in a reached node's distance of these types) and lists the extra ones; only those nodes
-    // take sigma sums and in-degrees. Wide workgroups (their survivor queues hold the lists), Brandes mode, f32
-    // distances (f64 ties are rare, and the kernels of the other types measured slower with the extra code); sigma
+    // the tree layers' costs: a scan of the internal nodes' tight arcs marks each head in an LDS bitmap and lists
+    // the heads it finds marked already; only those nodes take sigma sums and in-degrees. Wide workgroups (their
+    // survivor queues hold the lists), Brandes mode, f32 distances (f64 ties are rare, and the kernels of the other
+    // types measured slower with the extra code), LDS bounds (their region holds the bitmaps once verified); sigma
     // by order position goes to the second-dependency array, idle in this mode.
     constexpr bool kNear = MODE == MODE_BC && BS >= 512 && !LD && std::is_same_v<D, float>;
-    constexpr B kTop = static_cast<B>(B(1) << (8 * sizeof(B) - 1));
     const bool near_ok = kNear && use_bd && a.delta2 != nullptr;
     double* const sigp = near_ok ? a.delta2 + so : nullptr;
-    const B top = near_ok ? kTop : B(0);
     uint32_t* lists = a.lists + so * 3;
     uint32_t* const Fa = lists;
     uint32_t* const Fb = lists + stride;
@@ -1347,7 +1345,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
             c_logged += nlog;
             // arc e into t is tight when its candidate c equals the final distance of t
             uint32_t n_tight = 0;
-            auto tight = [&](uint32_t t, B c) -> bool { return !bd_skip(t, c) && c == (ldc(DP(t)) & ~top); };
+            auto tight = [&](uint32_t t, B c) -> bool { return !bd_skip(t, c) && c == ldc(DP(t)); };
             // Once the distances are final the settled-node bitmap is free; it becomes the map of the nodes with
             // a tight out-arc (`hc`). About two thirds to three quarters of a shortest-path tree's nodes are
             // leaves, and the layers below skip a leaf's row bounds, bitmap words and dependency record.
@@ -1364,13 +1362,6 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                     if ((wg_or(&bits[e >> 5], bit) & bit) == 0u) {
                         ++n_tight;
                         if (use_bm) atomicOr(&hc[u >> 5], 1u << (u & 31));
-                        if constexpr (kNear) {
-                            // the first tight in-arc of t sets the flag; each later one lists t once more
-                            if (near_ok && (wg_or(DP(t), top) & top) != B(0)) {
-                                const uint32_t k = atomicAdd(&S.cnt[7], 1u);
-                                if (k < Lds<B, BS>::kQ) S.q_t[k] = t;
-                            }
-                        }
                     }
                 }
             };
@@ -1396,7 +1387,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                             uint32_t t;
                             L l;
                             load_arc<L, BS>(tgt, len, arcs, e, t, l);
-                            fn(t, e, T::add(ldc(DP(v)) & ~top, l), v);
+                            fn(t, e, T::add(ldc(DP(v)), l), v);
                         });
                 }
             };
@@ -1422,18 +1413,47 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
             // ... and each internal node's number of children beside its own order position (in the first frontier
             // list): a leaf's dependency is 0, so its term 1 reaches the parent as part of that count, not as a push
             uint32_t* const nch = Fa;
-            // A near tree: its extra tight arcs (one per listed node entry) fit the list, and their nodes with the
-            // arcs into them fit the second list (at most twice as many entries). The bounds' LDS becomes the map of
-            // the nodes with several tight in-arcs (`multi`), whose in-degrees go to their records.
-            const uint32_t n_extra = S.cnt[7];
+            // A near tree: its extra tight arcs (every reached node but the source has one tight in-arc at least) fit
+            // the list, and the nodes with several tight in-arcs with the arcs into them fit the second list (at most
+            // twice as many entries). The bounds' LDS, free once verified, holds the bitmap of the heads seen by the
+            // scan below and then the map of the nodes with several tight in-arcs (`multi`), whose in-degrees go to
+            // their records.
+            const uint32_t n_extra = S.cnt[6] + 1u - nS;
             const bool near = kNear && near_ok && !tree && 2u * n_extra <= Lds<B, BS>::kQ;
             uint32_t* const multi = reinterpret_cast<uint32_t*>(fl);
+            uint32_t* const seen = multi + bmw;
             auto is_multi = [&](uint32_t v) -> bool { return ((multi[v >> 5] >> (v & 31)) & 1u) != 0u; };
             if constexpr (kNear) if (near) {
-                for (uint32_t w = tid; w < bmw; w += BS) multi[w] = 0u;
-                if (tid == 0) S.cnt[1] = 0;  // entries of the list of arcs into multi nodes (Kahn)
+                for (uint32_t w = tid; w < bmw; w += BS) {
+                    multi[w] = 0u;
+                    seen[w] = 0u;
+                }
+                if (tid == 0) {
+                    S.cnt[1] = 0;  // entries of the list of arcs into multi nodes (Kahn)
+                    S.cnt[7] = 0;  // listed heads
+                }
                 bsync();
-                for (uint32_t k = tid; k < n_extra; k += BS) {
+                // every tight arc once (the bitmap rows of the internal nodes): a head seen before is listed
+                for (uint32_t i = tid; i < nS; i += BS) {
+                    const uint32_t u = Sl[i];
+                    if (!internal(u)) continue;
+                    for_row_words(bits, off[u], off[u + 1], [&](uint32_t wi, uint32_t w) {
+                        while (w) {
+                            const uint32_t e = (wi << 5) + static_cast<uint32_t>(__ffs(w) - 1);
+                            w &= w - 1;
+                            const uint32_t t = target_of(e);
+                            const uint32_t bit = 1u << (t & 31);
+                            if ((atomicOr(&seen[t >> 5], bit) & bit) != 0u) {
+                                const uint32_t k = atomicAdd(&S.cnt[7], 1u);
+                                if (k < Lds<B, BS>::kQ) S.q_t[k] = t;
+                            }
+                        }
+                    });
+                }
+                bsync();
+                // the scan lists exactly n_extra heads (each tight arc beyond a node's first); bounded all the same
+                const uint32_t n_list = S.cnt[7] < Lds<B, BS>::kQ ? S.cnt[7] : Lds<B, BS>::kQ;
+                for (uint32_t k = tid; k < n_list; k += BS) {
                     const uint32_t t = S.q_t[k];
                     const uint32_t bit = 1u << (t & 31);
                     // in-degree = listed entries + the first tight in-arc
@@ -1732,7 +1752,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
         NG_TICK(t_reset);
         if (a.dist_out != nullptr) {
             for (uint32_t v = tid; v < n; v += BS)
-                write_dist_out(a, static_cast<size_t>(si) * n + v, ldc(DP(v)) & ~top);
+                write_dist_out(a, static_cast<size_t>(si) * n + v, ldc(DP(v)));
             bsync();  // every lane has read its distances before any lane resets them below
         }
         // ---- restore the slot: only reached nodes were touched (after an abort, frontier and pile nodes
