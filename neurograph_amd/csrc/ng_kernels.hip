@@ -952,7 +952,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
         // distance before it); new frontier entries are stamped `nph`. In the betweenness modes every candidate
         // that could be tight (nd <= old) is logged: light arcs by the frontier phases (a node is expanded again
         // after each improvement, so also with its final distance), heavy arcs by the settle passes.
-        auto apply_core = [&](bool live, uint32_t e, uint32_t t, B nd, B old, uint32_t nph, uint32_t u) {
+        auto apply_core = [&](bool live, uint32_t e, uint32_t t, B nd, B old, uint32_t nph, uint32_t u, bool heavy) {
             bool pf = false, pp = false, parked = false;
             if (nd < old) {
                 ++c_improve;
@@ -987,7 +987,10 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                 if (mask != 0ull) {
                     const uint32_t lane = lane_id();
                     const uint32_t rank = static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
-                    const uint32_t base = wave_reserve(&S.cnt[3], static_cast<uint32_t>(__popcll(mask)));
+                    // light arcs (frontier phases, possibly logged more than once) fill the log from the front,
+                    // heavy arcs (settle passes, each examined once per source) from the back
+                    const uint32_t base = wave_reserve(heavy ? &S.cnt[4] : &S.cnt[3], static_cast<uint32_t>(__popcll(mask)));
+                    const uint32_t pos = heavy ? a.logcap - 1u - (base + rank) : base + rank;
                     // the log is written once here and read once by the verification: non-temporal stores (and
                     // loads there), so its lines leave the caches first and the distances and rows stay longer
                     if (lg && base + rank < a.logcap) {
@@ -997,14 +1000,14 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                         q.y = e;
                         q.z = u;
                         q.w = static_cast<uint32_t>(ndu);
-                        __builtin_nontemporal_store(q, reinterpret_cast<u32x4*>(log + base + rank));
-                        if constexpr (sizeof(B) == 8) __builtin_nontemporal_store(static_cast<uint32_t>(ndu >> 32), log_hi + base + rank);
+                        __builtin_nontemporal_store(q, reinterpret_cast<u32x4*>(log + pos));
+                        if constexpr (sizeof(B) == 8) __builtin_nontemporal_store(static_cast<uint32_t>(ndu >> 32), log_hi + pos);
                     }
                 }
             }
         };
-        auto apply = [&](bool ok, uint32_t v, B dv, uint32_t e, uint32_t t, L l, bool final_t, B old, uint32_t nph) {
-            apply_core(ok && !final_t, e, t, T::add(dv, l), old, nph, v);
+        auto apply = [&](bool ok, uint32_t v, B dv, uint32_t e, uint32_t t, L l, bool final_t, B old, uint32_t nph, bool heavy) {
+            apply_core(ok && !final_t, e, t, T::add(dv, l), old, nph, v, heavy);
         };
         // the same steps for the compacted arc loop, given the candidate: the filter (heavy arc into a registered
         // node, or candidate above the target's bound) and the update
@@ -1047,12 +1050,12 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                 if constexpr (BS >= 512) {
                     expand_relax_compact<NG_PHASE_ARCS, BS>(
                         Fc, nF, S, tgt, len, arcs, light_row, [](bool, B) {}, [](B dv, L l) -> B { return T::add(dv, l); }, skip_arc, update_nd,
-                        [&](bool live, uint32_t e, uint32_t t, B nd, B old, uint32_t u) { apply_core(live, e, t, nd, old, nph, u); });
+                        [&](bool live, uint32_t e, uint32_t t, B nd, B old, uint32_t u) { apply_core(live, e, t, nd, old, nph, u, false); });
                 } else {
                     expand_relax<NG_UNROLL, BS>(
                         Fc, nF, S, tgt, len, arcs, light_row, [](bool, B) {}, filter, update,
                         [&](bool ok, uint32_t v, B dv, uint32_t e, uint32_t t, L l, bool fin, B old) {
-                            apply(ok, v, dv, e, t, l, fin, old, nph);
+                            apply(ok, v, dv, e, t, l, fin, old, nph, false);
                         });
                 }
                 // first visit in this bucket -> settled list
@@ -1162,12 +1165,12 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                     expand_relax_compact<kFilterArcs, BS>(
                         Sl + settle_from, nS - settle_from, S, tgt, len, arcs, settle_row, settle_tile,
                         [](B dv, L l) -> B { return T::add(dv, l); }, skip_arc, update_nd,
-                        [&](bool live, uint32_t e, uint32_t t, B nd, B old, uint32_t u) { apply_core(live, e, t, nd, old, nph, u); });
+                        [&](bool live, uint32_t e, uint32_t t, B nd, B old, uint32_t u) { apply_core(live, e, t, nd, old, nph, u, true); });
                 } else {
                     expand_relax<NG_UNROLL, BS>(
                         Sl + settle_from, nS - settle_from, S, tgt, len, arcs, settle_row, settle_tile, filter, update,
                         [&](bool ok, uint32_t v, B dv, uint32_t e, uint32_t t, L l, bool fin, B old) {
-                            apply(ok, v, dv, e, t, l, fin, old, nph);
+                            apply(ok, v, dv, e, t, l, fin, old, nph, true);
                         });
                 }
                 bsync();
@@ -1341,8 +1344,10 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
         if constexpr (MODE != MODE_SSSP) if (!aborted) {
             NG_TICK(t_verify);
             // ---- tight arcs: verify the logged candidates against the final distances
-            const uint32_t nlog = S.cnt[3];
-            c_logged += nlog;
+            const uint32_t nlog_l = S.cnt[3];  // light-arc entries [0, nlog_l)
+            const uint32_t nlog_h = S.cnt[4];  // heavy-arc entries [logcap - nlog_h, logcap)
+            const bool log_ok = static_cast<uint64_t>(nlog_l) + nlog_h <= a.logcap;
+            c_logged += static_cast<unsigned long long>(nlog_l) + nlog_h;
             // arc e into t is tight when its candidate c equals the final distance of t
             uint32_t n_tight = 0;
             auto tight = [&](uint32_t t, B c) -> bool { return !bd_skip(t, c) && c == ldc(DP(t)); };
@@ -1365,16 +1370,27 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                     }
                 }
             };
-            // every candidate (from the log, or from the full rows when the log overflowed) through fn
-            auto each_candidate = [&](auto fn) {
-                if (nlog <= a.logcap) {
-                    for (uint32_t i = tid; i < nlog; i += BS) {
+            // a heavy arc is logged at most once: its bit is set without waiting for the old word
+            auto mark_once = [&](uint32_t t, uint32_t e, B c, uint32_t u) {
+                if (tight(t, c)) {
+                    __hip_atomic_fetch_or(&bits[e >> 5], 1u << (e & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    ++n_tight;
+                    if (use_bm) atomicOr(&hc[u >> 5], 1u << (u & 31));
+                }
+            };
+            // every candidate (from the log: light entries through fn, heavy ones through fn_once; or from the full
+            // rows through fn when the log overflowed)
+            auto each_candidate = [&](auto fn, auto fn_once) {
+                if (log_ok) {
+                    auto entry = [&](uint32_t i, auto f) {
                         const u32x4 q = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(log + i));  // t, e, u, nd
                         unsigned long long nd = q.w;
                         if constexpr (sizeof(B) == 8)
                             nd |= static_cast<unsigned long long>(__builtin_nontemporal_load(log_hi + i)) << 32;
-                        fn(q.x, q.y, static_cast<B>(nd), q.z);
-                    }
+                        f(q.x, q.y, static_cast<B>(nd), q.z);
+                    };
+                    for (uint32_t i = tid; i < nlog_l; i += BS) entry(i, fn);
+                    for (uint32_t i = a.logcap - nlog_h + tid; i < a.logcap; i += BS) entry(i, fn_once);
                 } else {
                     expand<BS>(
                         Sl, nS, S,
@@ -1391,8 +1407,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                         });
                 }
             };
-            if (nlog > a.logcap && tid == 0) atomicOr(a.err, ERR_LOG_OVERFLOW);
-            each_candidate(mark);
+            if (!log_ok && tid == 0) atomicOr(a.err, ERR_LOG_OVERFLOW);
+            each_candidate(mark, mark_once);
             {
                 auto add32 = [](uint32_t x, uint32_t y) { return x + y; };
                 n_tight = wave_reduce(n_tight, add32);
