@@ -1494,6 +1494,56 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
             }
             NG_TOCK(t_verify, 4);
             NG_TICK(t_kahn);
+            // fn(i, v) for the order positions i in [lb, le) whose node v = order[i] is internal (has a tight out-arc). In
+            // wide workgroups each wave packs the internal positions of its share into its LDS queue and runs them 64
+            // at a time, so that its lanes all work on internal nodes (a quarter to a third of a tree's nodes) instead
+            // of waiting, one memory chain at a time, for the few lanes that hold one.
+            auto for_internal = [&](uint32_t lb, uint32_t le, auto fn) {
+                if constexpr (BS >= 512) {
+                    const uint32_t lane = lane_id();
+                    const unsigned long long below = (1ull << lane) - 1ull;
+                    uint32_t* const qi = S.q_t + (tid & ~63u);
+                    uint32_t* const qv = S.q_e + (tid & ~63u);
+                    uint32_t qc = 0;  // queued positions of this wave (wave-uniform)
+                    auto run = [&]() {
+                        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");  // the queue stores before its loads
+                        uint32_t i = 0, v = 0;
+                        const bool live = lane < qc;
+                        if (live) {
+                            i = qi[lane];
+                            v = qv[lane];
+                        }
+                        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");  // every lane has read before a refill
+                        if (live) fn(i, v);
+                        qc = 0;
+                    };
+                    for (uint32_t pb = lb + (tid & ~63u); pb < le; pb += BS) {  // wave-uniform trip count
+                        const uint32_t i = pb + lane;
+                        uint32_t v = 0;
+                        bool keep = false;
+                        if (i < le) {
+                            v = order[i];
+                            keep = internal(v);
+                        }
+                        const unsigned long long mk = __ballot(keep);
+                        const uint32_t c = static_cast<uint32_t>(__popcll(mk));
+                        if (c == 0u) continue;
+                        if (qc + c > 64u) run();
+                        if (keep) {
+                            const uint32_t q = qc + static_cast<uint32_t>(__popcll(mk & below));
+                            qi[q] = i;
+                            qv[q] = v;
+                        }
+                        qc += c;
+                    }
+                    if (qc != 0u) run();
+                } else {
+                    for (uint32_t i = lb + tid; i < le; i += BS) {
+                        const uint32_t v = order[i];
+                        if (internal(v)) fn(i, v);
+                    }
+                }
+            };
             // ---- Kahn layers over the tight DAG: sigma and a topological order
             if (tid == 0) {
                 order[0] = s;
@@ -1505,9 +1555,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
             while (lb < le && nl < n) {
                 if (tid == 0) layers[nl] = lb;
                 ++nl;
-                for (uint32_t i = lb + tid; i < le; i += BS) {
-                    const uint32_t u = order[i];
-                    if (!internal(u)) continue;  // a leaf releases no children
+                // one node u at order position i: a leaf (not internal) releases no children
+                auto kahn_node = [&](uint32_t i, uint32_t u) {
                     if (tree) {
                         // a tree node releases each tight child: up to four children's targets are loaded
                         // together and appended with one LDS atomic
@@ -1546,7 +1595,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                         });
                         if (k != 0u) flush();
                         if (push_dep) __builtin_nontemporal_store(kids, nch + i);
-                        continue;
+                        return;
                     }
                     if constexpr (kNear) if (near) {
                         // A near-tree node: sigma by order position (the source's is 1; a multi node's, marked -1, is
@@ -1585,7 +1634,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                             }
                         });
                         nch[i] = nsl;
-                        continue;
+                        return;
                     }
                     const double su = ldc(&R[u].sigma);
                     for_row_words(bits, off[u], off[u + 1], [&](uint32_t wi, uint32_t w) {
@@ -1601,6 +1650,14 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                             }
                         }
                     });
+                };
+                if (kNear && near) {
+                    for (uint32_t i = lb + tid; i < le; i += BS) {
+                        const uint32_t u = order[i];
+                        if (internal(u)) kahn_node(i, u);
+                    }
+                } else {
+                    for_internal(lb, le, kahn_node);
                 }
                 bsync();
                 lb = le;
@@ -1631,16 +1688,14 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                 rec_clean = le == nS;
                 for (int ly = static_cast<int>(nl) - 1; ly >= 1; --ly) {
                     const uint32_t l0 = layers[ly], l1 = layers[ly + 1];
-                    for (uint32_t i = l0 + tid; i < l1; i += BS) {
-                        const uint32_t t = __builtin_nontemporal_load(order + i);
-                        if (!internal(t)) continue;
+                    for_internal(l0, l1, [&](uint32_t i, uint32_t t) {
                         const uint32_t p = __builtin_nontemporal_load(par + i);
                         const double pushed = ldc(&R[t].delta1);
                         const double d = pushed + static_cast<double>(__builtin_nontemporal_load(nch + i));
                         R[t].bc += d;
                         if (rec_clean && pushed != 0.0) stc(&R[t].delta1, 0.0);
                         if (ly > 1) wg_add(&R[p].delta1, d);
-                    }
+                    });
                     bsync();
                 }
             } else if (kNear && near) {
@@ -1688,10 +1743,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                 rec_clean = tree && le == nS;
                 for (int ly = static_cast<int>(nl) - 1; ly >= 0; --ly) {
                     const uint32_t l0 = layers[ly], l1 = layers[ly + 1];
-                    for (uint32_t i = l0 + tid; i < l1; i += BS) {
-                        const uint32_t u = order[i];
-                        // a leaf has no tight children: its dependencies stay 0 and its bitmap row is clear
-                        if (!internal(u)) continue;
+                    // a leaf has no tight children: its dependencies stay 0 and its bitmap row is clear
+                    for_internal(l0, l1, [&](uint32_t i, uint32_t u) {
                         const double su = tree ? 1.0 : ldc(&R[u].sigma);
                         double acc1 = 0.0, acc2 = 0.0;
                         // the tight children in increasing arc order, two at a time: both children's loads are
@@ -1758,7 +1811,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                             }
                             if constexpr ((MODE & MODE_ARC2) != 0) if (!rec_clean || acc2 != 0.0) stc(&delta2[u], acc2);
                         }
-                    }
+                    });
                     bsync();
                 }
             }
