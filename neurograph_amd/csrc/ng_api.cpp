@@ -15,6 +15,7 @@
 #include <numeric>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "ng_internal.h"
@@ -227,6 +228,54 @@ bool row_in_order(const ng_graph* g, uint32_t u, const std::vector<L>& len, cons
     return true;
 }
 
+// Whether the arcs of finite length (lengths in the caller's arc order) connect every node to every other: a
+// search from node 0 along them and one against them both reach every node.
+template <class L>
+bool strongly_connected(const ng_graph* g, const std::vector<L>& len) {
+    const uint32_t n = g->n;
+    if (n <= 1) return true;
+    auto finite = [&](uint64_t a) -> bool {
+        if constexpr (std::is_floating_point_v<L>) return std::isfinite(len[a]);
+        else return true;
+    };
+    std::vector<uint8_t> seen(n, 0);
+    std::vector<uint32_t> stack;
+    stack.reserve(n);
+    auto search = [&](const uint64_t* off, const uint32_t* tgt, bool all) -> bool {
+        std::fill(seen.begin(), seen.end(), uint8_t(0));
+        seen[0] = 1;
+        stack.assign(1, 0u);
+        uint32_t reached = 1;
+        while (!stack.empty()) {
+            const uint32_t u = stack.back();
+            stack.pop_back();
+            for (uint64_t a = off[u]; a < off[u + 1]; ++a) {
+                if (all && !finite(a)) continue;
+                const uint32_t t = tgt[a];
+                if (!seen[t]) {
+                    seen[t] = 1;
+                    ++reached;
+                    stack.push_back(t);
+                }
+            }
+        }
+        return reached == n;
+    };
+    if (!search(g->off.data(), g->tgt.data(), true)) return false;
+    // the reversed arcs of finite length
+    std::vector<uint64_t> roff(static_cast<size_t>(n) + 1, 0);
+    for (uint32_t u = 0; u < n; ++u)
+        for (uint64_t a = g->off[u]; a < g->off[u + 1]; ++a)
+            if (finite(a)) ++roff[g->tgt[a] + 1];
+    for (uint32_t v = 0; v < n; ++v) roff[v + 1] += roff[v];
+    std::vector<uint32_t> rsrc(roff[n]);
+    std::vector<uint64_t> cur(roff.begin(), roff.end() - 1);
+    for (uint32_t u = 0; u < n; ++u)
+        for (uint64_t a = g->off[u]; a < g->off[u + 1]; ++a)
+            if (finite(a)) rsrc[cur[g->tgt[a]]++] = u;
+    return search(roff.data(), rsrc.data(), false);
+}
+
 // Host-side construction of the device CSR for |w| (vi = 0) or 1/|w| (vi = 1). Rows are sorted by (length,
 // target) so that the arcs lighter than the bucket width form a prefix. With the view already built in kind dk
 // (a weight update), its device arrays are reused: rows whose order the new lengths keep are not re-sorted, and
@@ -322,6 +371,12 @@ ng_status build_view_typed(ng_graph* g, int vi, DKind dk) {
         }
     } else {
         dfree(v.arcs);  // 8-byte lengths are read from tgt and len
+    }
+    {
+        // the arcs of finite length are those of before when every length was and is finite
+        const bool all_finite = fin == m;
+        if (!(update && all_finite && v.all_finite)) v.strong = strongly_connected<L>(g, len);
+        v.all_finite = all_finite;
     }
     v.dk = dk;
     v.min_len = fin ? mn : 0.0;
@@ -1476,7 +1531,7 @@ ng_status cohort_build(ng_cohort* c) {
         double d = pick_delta(g, v, 0, 1, c->delta[gi]);
         if (c->dk == DKind::U32 || c->dk == DKind::U64) d = std::max(1.0, std::round(d));
         eff_delta[gi] = d;
-        gtab[gi] = CohortGraph{v.off, v.tgt, v.len, v.arcs, nullptr, v.perm, g->n, 0u, d};
+        gtab[gi] = CohortGraph{v.off, v.tgt, v.len, v.arcs, nullptr, v.perm, g->n, v.strong ? 1u : 0u, d};
     }
     c->total = c->prefix[G];
     c->versions.resize(G);

@@ -73,6 +73,8 @@ struct View {
     int tuned_block[2] = {0, 0};         // lanes per workgroup per mode; 0 = not tuned
     double lend_delta = -1.0;
     double bound_inv_q = 0.0;  // scale of the 4-bit distance codes: 15 / (1.1 x largest sampled distance); 0 = not estimated
+    bool strong = false;       // the arcs of finite length connect every node to every other (strongly connected)
+    bool all_finite = false;   // every length is finite
     RowRange* rows = nullptr;  // [n] row bounds of each node with the end of its light prefix for lend_delta
     std::vector<uint32_t> perm_host;  // host copy of perm: weight updates re-sort only the rows whose order changes
     bool reordered_last = false;      // the last build or weight update changed the row order
@@ -158,8 +160,8 @@ struct CohortGraph {
     const RowRange* rows;
     const uint32_t* perm;
     uint32_t n;
-    uint32_t pad;
-    double delta;  // bucket width (integral for integer lengths)
+    uint32_t strong;  // 1: strongly connected (a source ends once every node is settled)
+    double delta;     // bucket width (integral for integer lengths)
 };
 
 struct LaunchArgs {

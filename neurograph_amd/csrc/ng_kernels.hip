@@ -593,6 +593,7 @@ struct KArgs {
     uint32_t* lists;
     void* pile;
     uint32_t pcap;
+    uint32_t strong;  // 1: the graph is strongly connected (a source ends once every node is settled)
     uint32_t* counters;
     uint32_t* order;
     uint32_t* layers;
@@ -818,6 +819,11 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
     uint32_t* ctr = a.counters + static_cast<size_t>(slot) * 4;
     uint32_t phase = ctr[0], gen = ctr[1], pgen = ctr[2];
     L delta = a.delta;
+    // On a strongly connected graph every node is reached: once all are settled, and the current bucket is
+    // [lo, lo + delta) with lo + delta > lo, the settle pass needs no heavy arcs (each candidate is at least hi,
+    // above every distance, so it can neither improve a distance nor be tight) and the pile holds only stale
+    // entries: the source ends without reading them.
+    uint32_t g_strong = a.strong;
     // work counters (uniform per workgroup, except the arc counts which lane 0 accumulates from the tiles)
     unsigned long long c_phases = 0, c_buckets = 0, c_logged = 0, c_layers = 0;
     unsigned long long c_trees = 0;  // sources whose tight arcs formed a shortest-path tree
@@ -861,6 +867,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                 rows = as_global(cg.rows);
                 perm = as_global(cg.perm);
                 delta = static_cast<L>(cg.delta);
+                g_strong = cg.strong;
                 bmw = use_bm ? (n + 31) / 32 : 0u;
                 cur_g = lo;
                 bsync();
@@ -1091,6 +1098,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
             // ---- settle pass: the heavy arcs of every settled node once, with its final distance
             const uint32_t nS = clampn(S.cnt[2]);
             if (aborted) break;
+            const bool finishing = g_strong != 0u && nS == n && skip_ok;  // every node settled: summary only
             if (sorted && nS > settle_from) {
                 uint32_t pos = settle_from;
                 for (uint32_t wb = 0; wb < bmw; wb += BS) {
@@ -1122,7 +1130,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                     dv = ldc(DP(v));
                     const u32x4 r = load_row(rows, v);  // x: first arc, y: light-prefix end, z: end
                     beg = r.y;
-                    cnt = r.z - beg;
+                    cnt = finishing ? 0u : r.z - beg;
                     c_settle += cnt;
                     deg = r.z - r.x;
                 };
@@ -1190,6 +1198,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                     continue;
                 }
             }
+            if (finishing) break;  // the pile holds only entries of settled nodes
             const uint32_t nP = S.cnt[1] > pcap ? (clampn(n + 1), 0u) : S.cnt[1];
             if (aborted || nP == 0) break;
             const bool exact = S.cnt[5] != 0u;
@@ -2007,6 +2016,7 @@ hipError_t launch_typed(const LaunchArgs& la, hipStream_t stream) {
         k.arcs = vw.arcs;
         k.rows = vw.rows;
         k.perm = vw.perm;
+        k.strong = vw.strong ? 1u : 0u;
     } else {
         k.gtab = la.gtab;
         k.n_graphs = la.n_graphs;
