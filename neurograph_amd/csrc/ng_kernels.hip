@@ -423,12 +423,14 @@ __device__ __forceinline__ bool above_bound(typename T::B c, uint16_t bd) {
     return T::val(c) > static_cast<double>(__uint_as_float(static_cast<uint32_t>(bd) << 16));
 }
 
-// Where 2 bytes per node do not fit, 4-bit codes: code(d) = floor(val(d) * inv_q) capped at 15 (15: no bound),
-// with the scale inv_q = 15 / (a little more than the graph's typical largest distance). A candidate c with
-// val(c) * inv_q >= code + 1 is above the distance: the rounded product is monotone in its argument, so
-// product(c) > product(d) implies c > d, and a code stored for an earlier, larger distance only weakens the
-// test. Two nodes share a byte; a byte store that races with its neighbour's may restore the neighbour's
-// earlier code, which is still a valid (weaker) bound.
+// Where 2 bytes per node do not fit, 4-bit codes: code(d) = floor(q(d)) capped at 15 (15: no bound), with
+// q(d) = (val(d) * inv_q)^2 / 15 and the scale inv_q = 15 / (a little more than the graph's typical largest
+// distance). The square spends the codes where the nodes are: most of them lie at the larger distances (a ball
+// grows with its radius), where the codes are finest (spatial200k: 11 % fewer probes than with codes linear in
+// the distance, 1.5 % less time). A candidate c with q(c) >= code + 1 is above the distance: the rounded products
+// are monotone in their argument, so q(c) > q(d) implies c > d, and a code stored for an earlier, larger distance
+// only weakens the test. Two nodes share a byte; a byte store that races with its neighbour's may restore the
+// neighbour's earlier code, which is still a valid (weaker) bound.
 __device__ __forceinline__ uint32_t code4(double x) { return x < 15.0 ? static_cast<uint32_t>(x) : 15u; }
 
 // Arc loop of the relaxation passes (settle passes and frontier phases) in wide workgroups, where few arcs
@@ -739,7 +741,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
         if (!use_bd) return false;
         if (bd4) {
             const uint32_t code = (static_cast<uint32_t>(fl8[t >> 1]) >> ((t & 1u) * 4u)) & 15u;
-            return code < 15u && T::val(c) * inv_q >= static_cast<double>(code + 1u);
+            const double y = T::val(c) * inv_q;
+            return code < 15u && y * y * (1.0 / 15.0) >= static_cast<double>(code + 1u);
         }
         return above_bound<T>(c, fl[t]);
     };
@@ -747,7 +750,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
     auto bd_set = [&](uint32_t t, B d) {
         if (!use_bd) return;
         if (bd4) {
-            const uint32_t code = code4(T::val(d) * inv_q);
+            const double y = T::val(d) * inv_q;
+            const uint32_t code = code4(y * y * (1.0 / 15.0));
             const uint32_t sh = (t & 1u) * 4u;
             const uint32_t old = fl8[t >> 1];
             if (code < ((old >> sh) & 15u)) fl8[t >> 1] = static_cast<uint8_t>((old & ~(15u << sh)) | (code << sh));
