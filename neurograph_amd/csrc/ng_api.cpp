@@ -580,6 +580,7 @@ ng_status ensure_ws(Workspace& w, hipStream_t st, uint32_t n, uint64_t m, uint32
             NG_HIP(hipMalloc(&w.rec, sizeof(BcRec) * std::max<size_t>(ns, 1)));
             NG_HIP(hipMalloc(&w.delta2, 8 * std::max<size_t>(ns, 1)));
             NG_HIP(launch_rec_init(w.rec, ns, st));
+            w.bc_zero = true;
             NG_HIP(launch_fill_f64(w.delta2, ns, 0.0, st));
             NG_HIP(hipMalloc(&w.order, 4 * std::max<size_t>(ns, 1)));
             NG_HIP(hipMalloc(&w.layers, 4 * static_cast<size_t>(slots) * (n + 1)));
@@ -691,7 +692,12 @@ ng_status run_batch(ng_graph* g, const RunSpec& rs, uint32_t* slots_used, double
     if (rs.n_sources) NG_HIP(hipMemcpyAsync(w.sources, rs.sources, sizeof(uint32_t) * rs.n_sources, hipMemcpyHostToDevice, s));
     NG_HIP(hipMemsetAsync(w.arcs_reached, 0, 8 * 32, s));
     NG_HIP(hipMemsetAsync(w.err, 0, 4, s));
-    if (rs.mode & MODE_BC) NG_HIP(launch_rec_zero_bc(w.rec, static_cast<uint64_t>(slots) * w.n, s));
+    if (rs.mode != MODE_SSSP) {
+        // every slot's betweenness partials start at zero: the last Brandes call's slot reduction cleared them, else
+        // they are cleared here (all slots, so that the flag covers them all); until this call's reduction they are not
+        if ((rs.mode & MODE_BC) && !w.bc_zero) NG_HIP(launch_rec_zero_bc(w.rec, static_cast<uint64_t>(w.slots) * w.n, s));
+        w.bc_zero = false;
+    }
     if (rs.mode & MODE_ARC1) NG_HIP(launch_fill_f64(w.grad1, g->m, 0.0, s));
     if (rs.mode & MODE_ARC2) NG_HIP(launch_fill_f64(w.grad2, g->m, 0.0, s));
     void* d_dist_out = rs.dist_dev;
@@ -1128,7 +1134,8 @@ ng_status ng_brandes_accumulate(ng_graph* g, const uint32_t* sources, uint32_t n
     if (st != NG_OK) return st;
     if (!n_sources) return NG_OK;
     Workspace& w = g->ctx->ws;
-    NG_HIP(launch_reduce_slots(w.rec, g->last.slots, g->n, w.n, w.bc_out, g->ctx->stream));
+    NG_HIP(launch_reduce_slots(w.rec, g->last.slots, g->n, w.n, w.bc_out, true, g->ctx->stream));
+    w.bc_zero = true;
     std::vector<double> tmp(g->n);
     NG_HIP(hipMemcpyAsync(tmp.data(), w.bc_out, 8 * static_cast<size_t>(g->n), hipMemcpyDeviceToHost, g->ctx->stream));
     NG_HIP(hipStreamSynchronize(g->ctx->stream));
@@ -1291,7 +1298,7 @@ ng_status ng_brandes_accumulate_noalloc(ng_graph* g, const uint32_t* sources, ui
         if (err & ERR_OVERFLOW) return fail(NG_EDEVICE, "queue overflow in the batched kernel");
         if (out_opt) NG_HIP(hipMemcpy(out_opt + b, w.summaries, sizeof(ng_sssp_summary) * k, hipMemcpyDeviceToHost));
     }
-    NG_HIP(launch_reduce_slots(w.rec, static_cast<uint32_t>(slots), n, n, w.bc_out, s));
+    NG_HIP(launch_reduce_slots(w.rec, static_cast<uint32_t>(slots), n, n, w.bc_out, false, s));
     NG_HIP(hipStreamSynchronize(s));
     // accumulate into the caller's vector through a fixed host buffer (no host allocation either)
     double part[4096];
@@ -1656,6 +1663,7 @@ ng_status cohort_replay(ng_cohort* c) {
         if (s0 != NG_OK) return s0;
     }
     NG_HIP(hipEventRecord(ctx->ev0, st));
+    ctx->ws.bc_zero = false;  // the replay adds betweenness partials of its own
     NG_HIP(hipGraphLaunch(c->exec, st));
     NG_HIP(hipEventRecord(ctx->ev1, st));
     const hipError_t se = hipStreamSynchronize(st);

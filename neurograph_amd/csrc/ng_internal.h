@@ -139,6 +139,7 @@ struct Workspace {
     size_t dist_bytes = 0;         // element size the dist array was allocated for
     DKind dist_kind = DKind::U32;  // distance kind the dist array currently holds +infinity of
     bool dist_valid = false;
+    bool bc_zero = false;          // every slot's betweenness partials (rec[].bc) are zero: no zeroing pass before a call
     uint64_t generation = 0;       // bumped whenever a buffer is reallocated (captured launches must re-capture)
 };
 
@@ -193,7 +194,8 @@ hipError_t launch_light_end(DKind dk, uint32_t n, const uint32_t* off, const voi
                             hipStream_t stream);
 // arcs[2k] = tgt[k], arcs[2k+1] = len[k] (4-byte lengths as bits), k < m
 hipError_t launch_interleave_arcs(const uint32_t* tgt, const uint32_t* len, uint64_t m, uint32_t* arcs, hipStream_t stream);
-hipError_t launch_reduce_slots(const BcRec* rec, uint32_t slots, uint32_t n, uint32_t stride, double* out, hipStream_t stream);
+// out[v] = sum over the slots of rec[slot][v].bc; with `zero`, each partial is set to 0 once read
+hipError_t launch_reduce_slots(BcRec* rec, uint32_t slots, uint32_t n, uint32_t stride, double* out, bool zero, hipStream_t stream);
 // Sets every record to the clean state (all zero).
 hipError_t launch_rec_init(BcRec* rec, uint64_t count, hipStream_t stream);
 // Zeroes the betweenness partial of every record.

@@ -1908,11 +1908,16 @@ __global__ void interleave_arcs_kernel(const uint32_t* tgt, const uint32_t* len,
 }
 
 // out[v] = sum over slots of the betweenness partial of node v, in slot order (deterministic)
-__global__ void reduce_slots_kernel(const BcRec* rec, uint32_t slots, uint32_t n, uint32_t stride, double* out) {
+// (with `zero`, each partial is cleared once read, so that the next call needs no zeroing pass of its own)
+__global__ void reduce_slots_kernel(BcRec* rec, uint32_t slots, uint32_t n, uint32_t stride, double* out, bool zero) {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
     double acc = 0.0;
-    for (uint32_t k = 0; k < slots; ++k) acc += rec[static_cast<size_t>(k) * stride + v].bc;
+    for (uint32_t k = 0; k < slots; ++k) {
+        double* const p = &rec[static_cast<size_t>(k) * stride + v].bc;
+        acc += *p;
+        if (zero) *p = 0.0;
+    }
     out[v] = acc;
 }
 
@@ -2284,9 +2289,9 @@ hipError_t launch_interleave_arcs(const uint32_t* tgt, const uint32_t* len, uint
     return hipGetLastError();
 }
 
-hipError_t launch_reduce_slots(const BcRec* rec, uint32_t slots, uint32_t n, uint32_t stride, double* out, hipStream_t stream) {
+hipError_t launch_reduce_slots(BcRec* rec, uint32_t slots, uint32_t n, uint32_t stride, double* out, bool zero, hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(reduce_slots_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, rec, slots, n, stride, out);
+    hipLaunchKernelGGL(reduce_slots_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, rec, slots, n, stride, out, zero);
     return hipGetLastError();
 }
 

@@ -170,6 +170,38 @@ def test_brandes_is_deterministic_and_accumulates(ctx):
     np.testing.assert_allclose(both, a, rtol=1e-12, atol=1e-12)
 
 
+def test_betweenness_partials_stay_clean_across_calls(ctx):
+    """Calls of every kind on one context's workspace, with different slot counts, in a row: each Brandes call starts
+    from clean per-slot betweenness partials (cleared by the previous Brandes call's slot reduction, or by a zeroing
+    pass after edge-dependency and adjoint calls and cohort replays, which leave partials of their own)."""
+    import neurograph_amd as ng
+
+    own = ng.Context(0)
+    c = rand_graph(2000, 16000, 61, wdtype=np.float64)
+    c2 = rand_graph(900, 7000, 62, wdtype=np.float64)
+    g = dev_graph(own, c)
+    g3 = dev_graph(own, c, max_slots=3)
+    h = dev_graph(own, c2)
+    srcs = np.arange(0, 2000, 9)
+    s2 = np.arange(0, 900, 5)
+    ref = orc.brandes(c, sources=srcs)["bc"]
+    ref2 = orc.brandes(c2, sources=s2)["bc"]
+    edge = orc.brandes(c, sources=srcs, edge=True)["edge"]
+    full2 = orc.brandes(c2, threads=8)["bc"]
+    co = ng.Cohort(own, [h])
+    for _ in range(2):
+        np.testing.assert_allclose(g.brandes(srcs), ref, rtol=1e-9, atol=1e-9)
+        np.testing.assert_allclose(g3.brandes(srcs), ref, rtol=1e-9, atol=1e-9)
+        np.testing.assert_allclose(g.brandes(srcs), ref, rtol=1e-9, atol=1e-9)
+        np.testing.assert_allclose(g.edge_dependency(srcs), edge, rtol=1e-9, atol=1e-12)
+        np.testing.assert_allclose(h.brandes(s2), ref2, rtol=1e-9, atol=1e-9)
+        g.adjoint(srcs)
+        np.testing.assert_allclose(g.brandes(srcs), ref, rtol=1e-9, atol=1e-9)
+        parts, _, _ = co.brandes()
+        np.testing.assert_allclose(parts[0], full2, rtol=1e-9, atol=1e-9)
+        np.testing.assert_allclose(h.brandes(s2), ref2, rtol=1e-9, atol=1e-9)
+
+
 def test_edge_dependency_and_adjoint_match_oracle(ctx):
     c = rand_graph(400, 3000, 31, wdtype=np.float64, parallel=False, self_loops=False)
     g = dev_graph(ctx, c)
