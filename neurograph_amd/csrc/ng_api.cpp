@@ -88,6 +88,9 @@ struct ng_context {
     int cus = 0;
     Workspace ws;
     hipError_t scratch_status = hipSuccess;  // result of resolving the device's scratch array at creation
+    // pinned staging of a call's betweenness vector on its way to the caller's (calls on a context are sequential)
+    double* host_bc = nullptr;
+    uint32_t host_bc_cap = 0;
     // the creator's handle plus one per live graph and cohort: ng_context_destroy releases the creator's share, and
     // the context is torn down when the last share goes, so graphs and cohorts may be destroyed after it
     std::atomic<uint32_t> refs{1};
@@ -103,6 +106,7 @@ void context_release(ng_context* ctx) {
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->host_bc) (void)hipHostFree(ctx->host_bc);
     free_ws(ctx->ws);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -1136,10 +1140,17 @@ ng_status ng_brandes_accumulate(ng_graph* g, const uint32_t* sources, uint32_t n
     Workspace& w = g->ctx->ws;
     NG_HIP(launch_reduce_slots(w.rec, g->last.slots, g->n, w.n, w.bc_out, true, g->ctx->stream));
     w.bc_zero = true;
-    std::vector<double> tmp(g->n);
-    NG_HIP(hipMemcpyAsync(tmp.data(), w.bc_out, 8 * static_cast<size_t>(g->n), hipMemcpyDeviceToHost, g->ctx->stream));
-    NG_HIP(hipStreamSynchronize(g->ctx->stream));
-    for (uint32_t v = 0; v < g->n; ++v) bc_inout[v] += tmp[v];
+    ng_context* const ctx = g->ctx;
+    if (ctx->host_bc_cap < g->n) {
+        if (ctx->host_bc) NG_HIP(hipHostFree(ctx->host_bc));
+        ctx->host_bc = nullptr;
+        ctx->host_bc_cap = 0;
+        NG_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->host_bc), 8 * static_cast<size_t>(g->n), hipHostMallocDefault));
+        ctx->host_bc_cap = g->n;
+    }
+    NG_HIP(hipMemcpyAsync(ctx->host_bc, w.bc_out, 8 * static_cast<size_t>(g->n), hipMemcpyDeviceToHost, ctx->stream));
+    NG_HIP(hipStreamSynchronize(ctx->stream));
+    for (uint32_t v = 0; v < g->n; ++v) bc_inout[v] += ctx->host_bc[v];
     return NG_OK;
 }
 
