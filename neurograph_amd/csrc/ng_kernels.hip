@@ -1233,7 +1233,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
             // split the pile: entries of settled nodes are dropped, values below hi join the frontier
             // (duplicates removed by the frontier stamp), the rest is carried over with its value. Each lane
             // has kSplit entries in flight; the minimum carried value is reduced per wave (one LDS atomic).
-            constexpr uint32_t kSplit = 4;
+            constexpr uint32_t kSplit = 2;
             B lmin = T::kInf;
             for (uint32_t base = 0; base < nP; base += kSplit * BS) {
                 PileEntry<B> pe[kSplit];
