@@ -1838,20 +1838,26 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
             bsync();  // every lane has read its distances before any lane resets them below
         }
         // ---- restore the slot: only reached nodes were touched (after an abort, frontier and pile nodes
-        // may hold distances too, so the whole array is reset)
-        if (aborted) {
+        // may hold distances too, so the whole array is reset). Where at least half of the nodes were reached,
+        // one coalesced pass over the whole distance array costs less than a scattered store per reached node.
+        const bool linear = aborted || 2ull * nS >= n;
+        if (linear) {
             for (uint32_t v = tid; v < n; v += BS) stc(DP(v), T::kInf);
         }
+        bool per_node = !linear;
         if constexpr (MODE != MODE_SSSP) {
             if (rec_clean && tid == 0) stc(&R[s].sigma, 0.0);  // on a tree only the source has a sigma
+            per_node = per_node || !rec_clean;
         }
-        for (uint32_t i = tid; i < nS; i += BS) {
-            const uint32_t v = Sl[i];
-            stc(DP(v), T::kInf);
-            if constexpr (MODE != MODE_SSSP) if (!rec_clean) {
-                stc(&R[v].sigma, 0.0);
-                stc(&R[v].delta1, 0.0);
-                if constexpr ((MODE & MODE_ARC2) != 0) stc(&delta2[v], 0.0);
+        if (per_node) {
+            for (uint32_t i = tid; i < nS; i += BS) {
+                const uint32_t v = Sl[i];
+                if (!linear) stc(DP(v), T::kInf);
+                if constexpr (MODE != MODE_SSSP) if (!rec_clean) {
+                    stc(&R[v].sigma, 0.0);
+                    stc(&R[v].delta1, 0.0);
+                    if constexpr ((MODE & MODE_ARC2) != 0) stc(&delta2[v], 0.0);
+                }
             }
         }
         bsync();
