@@ -1290,6 +1290,10 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
 
         NG_TICK(t_summary);
         const uint32_t nS = S.cnt[2] > n ? n : S.cnt[2];
+        // Where at least half of the nodes were reached, the tight-arc bitmap is cleared by one coalesced pass at
+        // the slot restore (every line of it holds tight arcs of some internal node) instead of an atomic and
+        // per word of each internal node's row where the Brandes layers read it.
+        const bool bits_linear = 2ull * nS >= n;
         // ---- per-source summary (SsspResult.h:43-64) over the reached nodes: the waves' sums of the settle passes,
         // or with LDS distances a pass over the reached nodes
         {
@@ -1595,7 +1599,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                             k = 0;
                         };
                         for_row_words(bits, off[u], off[u + 1], [&](uint32_t wi, uint32_t w) {
-                            if (push_dep) wg_and(&bits[wi], ~w);
+                            if (push_dep && !bits_linear) wg_and(&bits[wi], ~w);
                             while (w) {
                                 const uint32_t e = (wi << 5) + static_cast<uint32_t>(__ffs(w) - 1);
                                 w &= w - 1;
@@ -1627,7 +1631,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                             }
                         };
                         for_row_words(bits, off[u], off[u + 1], [&](uint32_t wi, uint32_t w) {
-                            wg_and(&bits[wi], ~w);
+                            if (!bits_linear) wg_and(&bits[wi], ~w);
                             while (w) {
                                 const uint32_t e = (wi << 5) + static_cast<uint32_t>(__ffs(w) - 1);
                                 w &= w - 1;
@@ -1807,7 +1811,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
                         uint32_t e_held = 0;
                         bool held = false;
                         for_row_words(bits, off[u], off[u + 1], [&](uint32_t wi, uint32_t w) {
-                            wg_and(&bits[wi], ~w);
+                            if (!bits_linear) wg_and(&bits[wi], ~w);
                             while (w) {
                                 const uint32_t e = (wi << 5) + static_cast<uint32_t>(__ffs(w) - 1);
                                 w &= w - 1;
@@ -1843,6 +1847,12 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS <= 256 ? 
         const bool linear = aborted || 2ull * nS >= n;
         if (linear) {
             for (uint32_t v = tid; v < n; v += BS) stc(DP(v), T::kInf);
+        }
+        if constexpr (MODE != MODE_SSSP) {
+            if (bits_linear) {
+                const uint32_t words = (off[n] + 31u) >> 5;
+                for (uint32_t w = tid; w < words; w += BS) stc(&bits[w], 0u);
+            }
         }
         bool per_node = !linear;
         if constexpr (MODE != MODE_SSSP) {
