@@ -93,3 +93,42 @@ def test_strongly_connected_edge_dependency_and_adjoint(ctx, wdtype):
     g1, g2 = g.adjoint(None, delta=0.4)
     np.testing.assert_allclose(g1, one["edge"], rtol=1e-9, atol=1e-12)
     np.testing.assert_allclose(g2, -inv["edge"], rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("block", [256, 1024])
+@pytest.mark.parametrize("wdtype", [np.int32, np.float32])
+def test_partial_reach_restores_slots_per_node(ctx, wdtype, block):
+    """Sources that reach fewer than half of the nodes restore their slot node by node (distances, tight-arc bitmap
+    words), the others by one coalesced pass over the arrays. Four disconnected components (each source reaches
+    under a quarter of the nodes), run again and again on the same slots and alternated with a connected graph on
+    the same workspace, keep every result the oracle's."""
+    import neurograph_amd as ng
+
+    own = ng.Context(0)
+    n4 = 900
+    rng = np.random.default_rng(77)
+    src, dst = [], []
+    for k in range(4):
+        base = k * n4
+        src.append(base + rng.integers(0, n4, size=8 * n4))
+        dst.append(base + rng.integers(0, n4, size=8 * n4))
+    src, dst = np.concatenate(src), np.concatenate(dst)
+    if np.issubdtype(wdtype, np.integer):
+        w = rng.integers(1, 10, size=len(src)).astype(wdtype)
+    else:
+        w = np.exp(rng.normal(0.0, 0.5, size=len(src))).astype(wdtype)
+    parts = orc.Csr.from_edges(4 * n4, src, dst, w, dtype=wdtype)
+    whole = strong_graph(4 * n4, 8 * 4 * n4, 78, wdtype)
+    gp = dev_graph(own, parts, block=block)
+    gw = dev_graph(own, whole, block=block)
+    sp = np.arange(0, 4 * n4, 23, dtype=np.uint32)
+    ref_p = orc.brandes(parts, sources=sp, want_dist=True)
+    ref_w = orc.brandes(whole, sources=sp, want_dist=True)
+    for _ in range(2):
+        for g, ref in ((gp, ref_p), (gw, ref_w), (gp, ref_p)):
+            _, dist = g.sssp(sp, want_dist=True)
+            np.testing.assert_array_equal(dist, ref["dist"])
+            np.testing.assert_allclose(g.brandes(sp), ref["bc"], rtol=1e-9, atol=1e-9)
+    d = ref_p["dist"]
+    top = np.finfo(d.dtype).max if d.dtype.kind == "f" else np.iinfo(d.dtype).max  # the oracle's "unreached"
+    assert (d == top).mean() > 0.7  # most nodes unreached from each source
